@@ -1,0 +1,134 @@
+/*
+ * lcs_hip.h -- C ABI of the MI355X (gfx950) semi-local LCS engine.
+ *
+ * The engine computes the seaweed (strand) permutation of two int32 strings
+ * with hand-written HIP kernels and merges permutations with a GPU steady
+ * ant.  Every entry point takes plain pointers and sizes, returns an int
+ * status (0 = OK, negative errno-style codes below) and never throws.
+ *
+ * Semantics follow egor-bv/platform-independent-lcs (paths relative to its
+ * root):
+ *   - a[0..m), b[0..n) int32 symbols, compared with ==;
+ *   - strands are int32 ids: h strands 0..m-1 enter the rows (row i holds
+ *     a[m-1-i]), v strands m..m+n-1 enter the columns (src/lcs_reference.hpp
+ *     :5-69);
+ *   - outputs h_out[i] (i < m) and v_out[j] (j < n) are the strands leaving
+ *     row i on the right and column j at the bottom; the permutation of size
+ *     m+n maps h_out[l] -> n+l and v_out[j] -> j (src/permutation_matrix.hpp
+ *     :88-115).
+ *
+ * Threading: a context is used by one host thread at a time, except the
+ * server entry points, which are internally synchronised.
+ */
+#ifndef LCS_HIP_H
+#define LCS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LCS_OK 0
+#define LCS_ENOENT (-2)     /* unknown algorithm / id                          */
+#define LCS_EIO (-5)        /* HIP runtime error                              */
+#define LCS_EAGAIN (-11)    /* result not ready yet / admission deferred      */
+#define LCS_ENOMEM (-12)    /* device memory or HBM budget exhausted          */
+#define LCS_EBUSY (-16)     /* object in use                                  */
+#define LCS_EINVAL (-22)    /* bad argument (null pointer, negative length)   */
+#define LCS_ERANGE (-34)    /* size beyond what the entry point supports      */
+#define LCS_ETIMEDOUT (-110)/* a device-side wait gave up (kernel aborted)    */
+#define LCS_ECANCELED (-125)/* request cancelled                              */
+
+typedef struct lcs_ctx lcs_ctx;
+
+/* A combing schedule.  Any field set to 0 means "choose automatically"
+ * (autotuner table, then the learned cost model, then the built-in default).
+ * Schedules only change speed: every schedule gives the same bits. */
+typedef struct lcs_schedule {
+    int32_t tile_m;          /* rows per lane: 1, 2, 4, 8, 16 or 32          */
+    int32_t waves_per_simd;  /* resident waves per SIMD the launch aims at   */
+    int32_t variant;         /* LCS_VARIANT_*                                */
+    int32_t strips;          /* column strips merged by the steady ant       */
+} lcs_schedule;
+
+#define LCS_VARIANT_AUTO 0
+#define LCS_VARIANT_CHAIN 1      /* wave64 band chain, registers only         */
+#define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the registry-owned sycl::queue (src/algorithm_registry.cpp:6-41,
+ * 167-173): owns the HIP stream, device workspaces and graphs.
+ * hbm_budget bounds the device memory the context may hold (0 = 3/4 of the
+ * free memory at creation). */
+int lcs_ctx_create(int device, size_t hbm_budget, lcs_ctx **out);
+int lcs_ctx_destroy(lcs_ctx *ctx);
+/* Name of the device the context runs on (the CSV `device` column of
+ * src/test_setup.hpp:331-347). */
+int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len);
+/* Number of compute units, and the kernel-measured time in milliseconds of
+ * the last lcs_comb / lcs_comb_device call (HIP events around the kernels). */
+int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms);
+
+/* ---- single pair --------------------------------------------------------
+ * Replaces an LcsProcedure registered in reg2 (src/lcs_interface_internal.hpp
+ * :91, src/algorithm_registry.cpp:149-155) such as Lcs_Semi_Tiled_MT
+ * (src/lcs_tiled.hpp:95-137) together with its residual fixup
+ * (src/lcs_grid.hpp:451-493): host pointers, synchronous.  h_out has m
+ * entries, v_out n entries; llcs_out (nullable) receives #{ j : v_out[j] < m }.
+ * sched may be NULL. */
+int lcs_comb(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+             const lcs_schedule *sched, int32_t *h_out, int32_t *v_out, int32_t *llcs_out);
+
+/* Same computation on device pointers, enqueued on `stream` (a hipStream_t;
+ * NULL = the context's stream).  Returns after enqueueing; the caller
+ * synchronises the stream.  Workspace comes from the context. */
+int lcs_comb_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
+                    const lcs_schedule *sched, int32_t *d_h_out, int32_t *d_v_out, void *stream);
+
+/* The residual fixup of the reference as a GPU operation: comb the
+ * sub-rectangle rows [i0, i0+isize) x columns [j0, j0+jsize) of the grid in
+ * place, starting from the strands currently held in h/v (host pointers,
+ * lengths m and n; a_prepared is a reversed, as in the reference's
+ * LcsProblemContext::a_prepared).  Follows Lcs_Semi_Fixup
+ * (src/lcs_residual_fixup.hpp:74-122). */
+int lcs_comb_fixup(lcs_ctx *ctx, const int32_t *a_prepared, int32_t m, const int32_t *b, int32_t n,
+                   int32_t *h, int32_t *v, int32_t i0, int32_t isize, int32_t j0, int32_t jsize);
+
+/* ---- permutations ------------------------------------------------------ */
+/* PermutationMatrix::FromStrands (src/permutation_matrix.hpp:88-115): host. */
+int lcs_permutation_from_strands(const int32_t *h, int32_t m, const int32_t *v, int32_t n,
+                                 int32_t *row_to_col, int32_t *col_to_row);
+/* PermutationMatrix::hash (src/permutation_matrix.hpp:192-203) with the
+ * reference's two's-complement wrap made well defined.  Host. */
+int64_t lcs_permutation_hash(const int32_t *col_to_row, int32_t size);
+
+/* staggered_multiply<RowGlue>(p, q, k) (src/braid_multiplication.hpp:273-406)
+ * on the GPU: the sticky-braid product of p (size p_size) and q (size
+ * q_size) glued on k strands; row_glue = 0 glues p's last k columns to q's
+ * first k rows (b-split), 1 glues p's first k columns to q's last k rows
+ * (a-split).  Host pointers (row_to_col form); out has p_size+q_size-k
+ * entries. */
+int lcs_braid_multiply(lcs_ctx *ctx, const int32_t *p_row_to_col, int32_t p_size,
+                       const int32_t *q_row_to_col, int32_t q_size, int32_t k, int32_t row_glue,
+                       int32_t *out_row_to_col);
+
+/* ---- batched pairs ------------------------------------------------------
+ * No counterpart in the reference (it combs one pair per call); the batch
+ * form the reference's harness would call once per line of its run script
+ * (src/main.cpp:46-64).  Pair p is a[a_off[p] .. a_off[p]+m[p]) against
+ * b[b_off[p] .. b_off[p]+n[p]) inside two concatenated host arrays; outputs
+ * go to h_out[a_off[p]..] and v_out[b_off[p]..].  scheds (nullable) gives a
+ * schedule per pair.  use_graph != 0 captures the launch in a hipGraph. */
+int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs,
+                   const int32_t *a_all, const int64_t *a_off, const int32_t *m,
+                   const int32_t *b_all, const int64_t *b_off, const int32_t *n,
+                   const lcs_schedule *scheds, int32_t use_graph,
+                   int32_t *h_out, int32_t *v_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCS_HIP_H */

@@ -1,0 +1,46 @@
+// Internal state of a context (shared by the C ABI translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "lcs_hip.h"
+
+namespace lcs {
+
+// A grow-only device buffer whose bytes count against the context's budget.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(lcs_ctx *ctx, size_t want);
+    void release(lcs_ctx *ctx);
+};
+
+struct Extras;  // batch / braid / server resources, created on first use
+
+}  // namespace lcs
+
+struct lcs_ctx {
+    int device = 0;
+    int num_cus = 0;
+    std::string name, arch;
+    size_t budget = 0;
+    size_t held = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned *ctrl = nullptr;            // [0] ticket, [1] abort flag, rest spare
+    unsigned long long timeout_ticks = 500000000ull;  // 5 s at 100 MHz
+    double last_kernel_ms = 0.0;
+    bool pending_timing = false;
+    lcs::DevBuf ring, in_a, in_b, out_h, out_v;
+    lcs::Extras *extras = nullptr;
+};
+
+namespace lcs {
+int default_tile_m(const lcs_ctx *ctx, int32_t m);
+int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
+                  const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
+                  const int32_t *d_h_in = nullptr, const int32_t *d_v_in = nullptr);
+int check_abort(lcs_ctx *ctx);
+void destroy_extras(lcs_ctx *ctx);
+}  // namespace lcs

@@ -1,0 +1,289 @@
+"""Python bindings (ctypes) of the MI355X semi-local LCS engine.
+
+The engine itself is the C++/HIP library ``lib/liblcs_hip.so`` built from
+``csrc/`` for gfx950; this module only marshals numpy arrays through its C ABI
+(``include/lcs_hip.h``).  There is no fallback: when the library is missing
+or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "liblcs_hip.so")
+
+LCS_OK = 0
+ERRORS = {
+    -2: "ENOENT", -5: "EIO", -11: "EAGAIN", -12: "ENOMEM", -16: "EBUSY", -22: "EINVAL",
+    -34: "ERANGE", -110: "ETIMEDOUT", -125: "ECANCELED",
+}
+
+VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS = 0, 1, 2
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class LcsError(RuntimeError):
+    def __init__(self, fn: str, rc: int):
+        super().__init__(f"{fn} failed: {rc} ({ERRORS.get(rc, '?')})")
+        self.rc = rc
+
+
+class Schedule(ctypes.Structure):
+    _fields_ = [("tile_m", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32),
+                ("variant", ctypes.c_int32), ("strips", ctypes.c_int32)]
+
+    def __repr__(self):
+        return (f"Schedule(tile_m={self.tile_m}, waves_per_simd={self.waves_per_simd}, "
+                f"variant={self.variant}, strips={self.strips})")
+
+
+def build() -> None:
+    """Compile the HIP library in-tree (hipcc cross-compiles for gfx950)."""
+    subprocess.run(["make", "-s", "-j8", "-C", PKG_ROOT], check=True)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with make -C {PKG_ROOT}")
+    L = ctypes.CDLL(LIB_PATH)
+    i32, i64, vp, sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_size_t
+    sched_p = ctypes.POINTER(Schedule)
+    L.lcs_ctx_create.argtypes = [ctypes.c_int, sz, ctypes.POINTER(vp)]
+    L.lcs_ctx_destroy.argtypes = [vp]
+    L.lcs_ctx_device_name.argtypes = [vp, ctypes.c_char_p, sz]
+    L.lcs_ctx_info.argtypes = [vp, _i32p, ctypes.POINTER(ctypes.c_double)]
+    L.lcs_comb.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
+    L.lcs_comb_device.argtypes = [vp, vp, i32, vp, i32, sched_p, vp, vp, vp]
+    L.lcs_comb_fixup.argtypes = [vp, _i32p, i32, _i32p, i32, _i32p, _i32p, i32, i32, i32, i32]
+    L.lcs_permutation_from_strands.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p]
+    L.lcs_permutation_hash.argtypes = [_i32p, i32]
+    L.lcs_permutation_hash.restype = i64
+    L.lcs_braid_multiply.argtypes = [vp, _i32p, i32, _i32p, i32, i32, i32, _i32p]
+    L.lcs_batch_comb.argtypes = [vp, i32, _i32p, _i64p, _i32p, _i32p, _i64p, _i32p, sched_p, i32, _i32p, _i32p]
+    # workloads
+    L.lcs_gen_binary_mt19937.argtypes = [i32, i32, _i32p]
+    L.lcs_gen_uniform.argtypes = [ctypes.c_uint64, i32, i32, _i32p]
+    L.lcs_gen_zipf.argtypes = [ctypes.c_uint64, i32, i32, ctypes.c_double, _i32p]
+    L.lcs_gen_mutate.argtypes = [ctypes.c_uint64, _i32p, i32, i32, ctypes.c_double, ctypes.c_double, i32, _i32p]
+    L.lcs_gen_planted.argtypes = [ctypes.c_uint64, _i32p, i32, i32, i32, ctypes.c_double, i32,
+                                  ctypes.c_double, _i32p]
+    L.lcs_gen_unit.argtypes = [ctypes.c_uint64, i64]
+    L.lcs_gen_unit.restype = ctypes.c_double
+    _lib = L
+    return L
+
+
+def _p(x: np.ndarray):
+    return x.ctypes.data_as(_i32p)
+
+
+def _i32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _check(fn: str, rc: int) -> None:
+    if rc != LCS_OK:
+        raise LcsError(fn, rc)
+
+
+def schedule(tile_m=0, waves_per_simd=0, variant=0, strips=0) -> Schedule:
+    return Schedule(tile_m, waves_per_simd, variant, strips)
+
+
+class Context:
+    """An engine context on one GPU (lcs_ctx_create / lcs_ctx_destroy)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._h = ctypes.c_void_p()
+        _check("lcs_ctx_create", lib().lcs_ctx_create(device, hbm_budget, ctypes.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().lcs_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check("lcs_ctx_device_name", lib().lcs_ctx_device_name(self._h, buf, 256))
+        return buf.value.decode()
+
+    def info(self):
+        cus = ctypes.c_int32()
+        ms = ctypes.c_double()
+        _check("lcs_ctx_info", lib().lcs_ctx_info(self._h, ctypes.byref(cus), ctypes.byref(ms)))
+        return cus.value, ms.value
+
+    def comb(self, a, b, sched: Schedule | None = None):
+        """(h_out, v_out, llcs) of the seaweed comb of a against b."""
+        a, b = _i32(a), _i32(b)
+        h = np.empty(max(a.size, 1), np.int32)
+        v = np.empty(max(b.size, 1), np.int32)
+        llcs = ctypes.c_int32(-1)
+        rc = lib().lcs_comb(self._h, _p(a), a.size, _p(b), b.size,
+                            ctypes.byref(sched) if sched is not None else None,
+                            _p(h), _p(v), ctypes.byref(llcs))
+        _check("lcs_comb", rc)
+        return h[: a.size].copy(), v[: b.size].copy(), llcs.value
+
+    def comb_fixup(self, a_prepared, b, h, v, i0, isize, j0, jsize):
+        ap, b = _i32(a_prepared), _i32(b)
+        assert h.dtype == np.int32 and v.dtype == np.int32 and h.flags.c_contiguous and v.flags.c_contiguous
+        _check("lcs_comb_fixup", lib().lcs_comb_fixup(self._h, _p(ap), ap.size, _p(b), b.size, _p(h), _p(v),
+                                                      i0, isize, j0, jsize))
+
+    def braid_multiply(self, p_r2c, q_r2c, k: int, row_glue: bool) -> np.ndarray:
+        p, q = _i32(p_r2c), _i32(q_r2c)
+        out = np.empty(max(p.size + q.size - k, 1), np.int32)
+        _check("lcs_braid_multiply", lib().lcs_braid_multiply(self._h, _p(p), p.size, _p(q), q.size, k,
+                                                              int(bool(row_glue)), _p(out)))
+        return out[: p.size + q.size - k].copy()
+
+    def batch_comb(self, a_list, b_list, scheds=None, use_graph: bool = True):
+        """Comb every pair (a_list[p], b_list[p]) in one batched launch."""
+        npairs = len(a_list)
+        m = np.array([len(x) for x in a_list], np.int32)
+        n = np.array([len(x) for x in b_list], np.int32)
+        a_off = np.zeros(npairs, np.int64)
+        b_off = np.zeros(npairs, np.int64)
+        if npairs:
+            a_off[1:] = np.cumsum(m[:-1], dtype=np.int64)
+            b_off[1:] = np.cumsum(n[:-1], dtype=np.int64)
+        a_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in a_list]) if npairs else [])
+        b_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in b_list]) if npairs else [])
+        h = np.empty(max(a_all.size, 1), np.int32)
+        v = np.empty(max(b_all.size, 1), np.int32)
+        sarr = None
+        if scheds is not None:
+            sarr = (Schedule * npairs)(*scheds)
+        rc = lib().lcs_batch_comb(self._h, npairs, _p(a_all), a_off.ctypes.data_as(_i64p), _p(m),
+                                  _p(b_all), b_off.ctypes.data_as(_i64p), _p(n), sarr, int(use_graph),
+                                  _p(h), _p(v))
+        _check("lcs_batch_comb", rc)
+        return ([h[a_off[p]: a_off[p] + m[p]].copy() for p in range(npairs)],
+                [v[b_off[p]: b_off[p] + n[p]].copy() for p in range(npairs)])
+
+
+def from_strands(h, v):
+    h, v = _i32(h), _i32(v)
+    size = h.size + v.size
+    r2c = np.empty(max(size, 1), np.int32)
+    c2r = np.empty(max(size, 1), np.int32)
+    _check("lcs_permutation_from_strands",
+           lib().lcs_permutation_from_strands(_p(h), h.size, _p(v), v.size, _p(r2c), _p(c2r)))
+    return r2c[:size].copy(), c2r[:size].copy()
+
+
+def perm_hash(col_to_row) -> int:
+    c2r = _i32(col_to_row)
+    return int(lib().lcs_permutation_hash(_p(c2r), c2r.size))
+
+
+# ---------------------------------------------------------------- workloads
+def gen_binary(seed: int, length: int) -> np.ndarray:
+    out = np.empty(max(length, 1), np.int32)
+    lib().lcs_gen_binary_mt19937(seed, length, _p(out))
+    return out[:length].copy()
+
+
+def gen_uniform(seed: int, length: int, alphabet: int) -> np.ndarray:
+    out = np.empty(max(length, 1), np.int32)
+    lib().lcs_gen_uniform(seed, length, alphabet, _p(out))
+    return out[:length].copy()
+
+
+def gen_zipf(seed: int, length: int, alphabet: int, s: float) -> np.ndarray:
+    out = np.empty(max(length, 1), np.int32)
+    lib().lcs_gen_zipf(seed, length, alphabet, s, _p(out))
+    return out[:length].copy()
+
+
+def gen_mutate(seed: int, src, alphabet: int, sub_rate: float, indel_rate: float, out_len: int) -> np.ndarray:
+    src = _i32(src)
+    out = np.empty(max(out_len, 1), np.int32)
+    lib().lcs_gen_mutate(seed, _p(src), src.size, alphabet, sub_rate, indel_rate, out_len, _p(out))
+    return out[:out_len].copy()
+
+
+def gen_planted(seed: int, a, n: int, alphabet: int, s: float, copies: int, sub_rate: float) -> np.ndarray:
+    a = _i32(a)
+    out = np.empty(max(n, 1), np.int32)
+    lib().lcs_gen_planted(seed, _p(a), a.size, n, alphabet, s, copies, sub_rate, _p(out))
+    return out[:n].copy()
+
+
+def gen_unit(seed: int, k: int) -> float:
+    return float(lib().lcs_gen_unit(seed, k))
+
+
+@dataclass
+class Pair:
+    a: np.ndarray
+    b: np.ndarray
+    alphabet: int
+    rate: float = 0.0
+
+
+def workload_single(name: str, seed: int = 1) -> Pair:
+    """The single-pair workloads: 'uniform4096', 'mutated2^20', 'zipf_planted'."""
+    if name == "uniform4096":
+        return Pair(gen_uniform(seed, 4096, 4), gen_uniform(seed + 1, 4096, 4), 4)
+    if name.startswith("mutated"):
+        size = int(eval(name[len("mutated"):].replace("^", "**")))
+        a = gen_uniform(seed, size, 4)
+        b = gen_mutate(seed + 1, a, 4, 0.10, 0.02, size)
+        return Pair(a, b, 4, 0.12)
+    if name == "zipf_planted":
+        a = gen_zipf(seed, 1 << 14, 26, 1.1)
+        b = gen_planted(seed + 1, a, 1 << 22, 26, 1.1, 64, 0.05)
+        return Pair(a, b, 26, 0.05)
+    raise ValueError(name)
+
+
+def workload_batch(npairs: int, seed: int = 7, lo: int = 256, hi: int = 16384, alphabet=4,
+                   rate_lo: float = 0.05, rate_hi: float = 0.20, indel_frac: float = 0.2):
+    """Pairs with m, n drawn independently log-uniform in [lo, hi]; B derived
+    from A (cyclically extended or cropped to n) with a per-pair mutation
+    rate uniform in [rate_lo, rate_hi] of which indel_frac are indels.
+    `alphabet` is an int or a tuple to draw from uniformly."""
+    pairs = []
+    lg_lo, lg_hi = np.log(lo), np.log(hi)
+    for p in range(npairs):
+        u = [gen_unit(seed, 4 * p + q) for q in range(4)]
+        m = int(np.floor(np.exp(lg_lo + (lg_hi - lg_lo) * u[0])))
+        n = int(np.floor(np.exp(lg_lo + (lg_hi - lg_lo) * u[1])))
+        m, n = min(max(m, lo), hi), min(max(n, lo), hi)
+        rate = rate_lo + (rate_hi - rate_lo) * u[2]
+        alpha = alphabet if isinstance(alphabet, int) else alphabet[int(u[3] * len(alphabet)) % len(alphabet)]
+        a = gen_uniform(seed * 1000003 + 2 * p, m, alpha)
+        b = gen_mutate(seed * 1000003 + 2 * p + 1, a, alpha, rate * (1 - indel_frac), rate * indel_frac, n)
+        pairs.append(Pair(a, b, alpha, rate))
+    return pairs

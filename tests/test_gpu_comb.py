@@ -1,0 +1,103 @@
+"""Parity of the HIP band-chain comb with the CPU oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+import lcs_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(ctx, a, b, sched=None):
+    h, v, llcs = ctx.comb(a, b, sched)
+    ho, vo = oracle.comb(a, b)
+    assert np.array_equal(h, ho), f"h mismatch m={len(a)} n={len(b)} sched={sched}"
+    assert np.array_equal(v, vo), f"v mismatch m={len(a)} n={len(b)} sched={sched}"
+    assert llcs == oracle.llcs_from_strands(vo, len(a))
+    return h, v
+
+
+def test_tiny_known_answer(ctx):
+    h, v, llcs = ctx.comb([1, 2], [1])
+    assert list(h) == [0, 2] and list(v) == [1] and llcs == 1
+
+
+@pytest.mark.parametrize("m,n", [(0, 0), (0, 5), (7, 0), (1, 1), (1, 100), (100, 1), (63, 64), (64, 63),
+                                 (65, 129), (300, 17), (17, 300)])
+def test_edge_shapes(ctx, m, n):
+    rng = np.random.default_rng(m * 1000 + n)
+    a = rng.integers(0, 3, m).astype(np.int32)
+    b = rng.integers(0, 3, n).astype(np.int32)
+    for tm in (1, 4, 16):
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
+
+
+@pytest.mark.parametrize("tm", [1, 2, 4, 8, 16, 32])
+def test_tile_heights(ctx, tm):
+    rng = np.random.default_rng(tm)
+    for m, n in [(1000, 1500), (4096, 777), (5000, 64 * 32 + 5)]:
+        a = rng.integers(0, 4, m).astype(np.int32)
+        b = rng.integers(0, 4, n).astype(np.int32)
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
+
+
+def test_uniform4096(ctx):
+    w = lcs_amd.workload_single("uniform4096")
+    for tm in (0, 1, 2, 4):
+        _check(ctx, w.a, w.b, lcs_amd.schedule(tile_m=tm))
+
+
+@pytest.mark.parametrize("m,n", [(7330, 9944), (9771, 11302)])
+def test_reference_script_sizes(ctx, m, n):
+    # non-aligned sizes of the reference's run script (test_default.txt)
+    a = lcs_amd.gen_binary(888, m)
+    b = lcs_amd.gen_binary(999, n)
+    _check(ctx, a, b)
+    _check(ctx, a, b, lcs_amd.schedule(tile_m=8, waves_per_simd=2))
+
+
+def test_large_alphabet_and_negative_symbols(ctx):
+    rng = np.random.default_rng(5)
+    a = rng.integers(-2**31, 2**31 - 1, 3000, dtype=np.int64).astype(np.int32)
+    b = rng.integers(-2**31, 2**31 - 1, 2000, dtype=np.int64).astype(np.int32)
+    b[::7] = a[: len(b[::7])]
+    _check(ctx, a, b)
+
+
+def test_all_match_and_no_match(ctx):
+    a = np.zeros(2000, np.int32)
+    h, v, llcs = ctx.comb(a, np.zeros(1500, np.int32))
+    assert llcs == 1500
+    h, v, llcs = ctx.comb(a, np.ones(1500, np.int32))
+    assert llcs == 0 and np.array_equal(h, np.arange(2000)) and np.array_equal(v, 2000 + np.arange(1500))
+
+
+def test_mutated_pair_and_hash(ctx):
+    w = lcs_amd.workload_single("mutated2^15")
+    h, v = _check(ctx, w.a, w.b)
+    r2c, c2r = lcs_amd.from_strands(h, v)
+    assert lcs_amd.perm_hash(c2r) == oracle.perm_hash(oracle.from_strands(h, v)[1])
+
+
+def test_fixup_quadrants(ctx):
+    rng = np.random.default_rng(11)
+    m, n = 700, 900
+    a = rng.integers(0, 4, m).astype(np.int32)
+    b = rng.integers(0, 4, n).astype(np.int32)
+    ap = a[::-1].copy()
+    h = np.arange(m, dtype=np.int32)
+    v = (m + np.arange(n)).astype(np.int32)
+    # reference order: rows with high i first (they feed v down), then the rest
+    i_split, j_split = 300, 450
+    ctx.comb_fixup(ap, b, h, v, i_split, m - i_split, 0, j_split)
+    ctx.comb_fixup(ap, b, h, v, 0, i_split, 0, j_split)
+    ctx.comb_fixup(ap, b, h, v, 0, m, j_split, n - j_split)
+    ho, vo = oracle.comb(a, b)
+    assert np.array_equal(h, ho) and np.array_equal(v, vo)
+
+
+def test_repeatable(ctx):
+    w = lcs_amd.workload_single("mutated2^14")
+    r1 = ctx.comb(w.a, w.b)
+    r2 = ctx.comb(w.a, w.b, lcs_amd.schedule(tile_m=4, waves_per_simd=2))
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
