@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cell_asm.h"
 #include "comb_kernels.h"
 
 namespace lcsk {
@@ -92,6 +93,30 @@ __device__ __forceinline__ void cell(int32_t sa, int32_t sb, bool valid, uint32_
     h = hn;
 }
 
+// All TM rows of one column for one lane.
+template <int TM, bool ROWMASK>
+__device__ __forceinline__ void column_cells(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
+                                             int32_t sb, uint32_t &v)
+{
+    // Rows go through hand-scheduled blocks of up to 16 (cell_asm.h); one
+    // block per column keeps the whole chain free of hazard nops.
+    if constexpr (!ROWMASK && TM >= 4) {
+        constexpr int R = TM >= 16 ? 16 : TM;
+#pragma unroll
+        for (int k = 0; k < TM; k += R) {
+            uint32_t hn[R];
+            if constexpr (R == 16) cell_block16(&a[k], &h[k], hn, sb, v);
+            else if constexpr (R == 8) cell_block8(&a[k], &h[k], hn, sb, v);
+            else cell_block4(&a[k], &h[k], hn, sb, v);
+#pragma unroll
+            for (int q = 0; q < R; ++q) h[k + q] = hn[q];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < TM; ++k) cell<ROWMASK>(a[k], sb, valid[k], h[k], v);
+    }
+}
+
 // Steps t0 .. t0+C-1 of a band.  COLMASK: some lane's column leaves [0, n).
 template <int TM, int C, bool ROWMASK, bool COLMASK>
 __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
@@ -113,10 +138,7 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
             const int j = t0 + s - lane;
             active = (j >= 0) && (j < n);
         }
-        if (active) {
-#pragma unroll
-            for (int k = 0; k < TM; ++k) cell<ROWMASK>(a[k], bq, valid[k], h[k], v);
-        }
+        if (active) column_cells<TM, ROWMASK>(a, h, valid, bq, v);
     }
 }
 

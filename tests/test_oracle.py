@@ -1,0 +1,181 @@
+"""Pins of the CPU oracle (runs on the CPU).
+
+The reference holds no golden vectors or known-answer tests for this path,
+so the oracle is pinned by brute force and closed forms: the textbook LCS
+DP, the semi-local scores of every window, hand-derived tiny grids, the
+composition law of the braid product against the comb, the definition of
+the distance product, the reference's own PermutationMatrix compiled from
+its source (oracle/_ref) and a big-integer restatement of its hash.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rnd(seed, m, n, sigma):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, sigma, m).astype(np.int32), rng.integers(0, sigma, n).astype(np.int32)
+
+
+def test_tiny_known_answers():
+    # a={1,2}, b={1}: row i=0 holds a[1]=2, row i=1 holds a[0]=1 (reversed).
+    # Cell (1,0): 1==1 -> swap h=1,v=2 -> h[1]=2, v=1.  Cell (0,0): 2 vs 1
+    # mismatch, h=0 > v=1? no -> keep.  So h=[0,2], v=[1].
+    h, v = oracle.comb([1, 2], [1])
+    assert list(h) == [0, 2] and list(v) == [1]
+    # single matching cell swaps, single mismatching cell keeps
+    assert [list(x) for x in oracle.comb([5], [5])] == [[1], [0]]
+    assert [list(x) for x in oracle.comb([5], [6])] == [[0], [1]]
+
+
+def test_no_match_is_identity():
+    a = np.zeros(37, np.int32)
+    b = np.ones(23, np.int32)
+    h, v = oracle.comb(a, b)
+    assert list(h) == list(range(37)) and list(v) == list(range(37, 60))
+
+
+@pytest.mark.parametrize("order", ["rows", "antidiag", "mt"])
+def test_llcs_equals_dp(order):
+    for seed in range(60):
+        rng = np.random.default_rng(seed)
+        m, n, s = rng.integers(0, 40), rng.integers(0, 40), rng.integers(1, 6)
+        a, b = rnd(seed, m, n, s)
+        h, v = oracle.comb(a, b, order=order, threads=3)
+        assert oracle.llcs_from_strands(v, m) == oracle.llcs_dp(a, b)
+        r2c, c2r = oracle.from_strands(h, v)
+        assert sorted(r2c) == list(range(m + n))
+
+
+def test_equal_strings_and_orders_agree():
+    a, b = rnd(3, 500, 700, 4)
+    assert oracle.llcs_from_strands(oracle.comb(a, a)[1], 500) == 500
+    r = oracle.comb(a, b, "rows")
+    for order in ("antidiag", "mt"):
+        o = oracle.comb(a, b, order, threads=5)
+        assert np.array_equal(r[0], o[0]) and np.array_equal(r[1], o[1])
+
+
+def test_semilocal_windows_equal_dp():
+    for seed in range(25):
+        rng = np.random.default_rng(100 + seed)
+        m, n, s = rng.integers(1, 25), rng.integers(1, 25), rng.integers(1, 4)
+        a, b = rnd(seed, m, n, s)
+        r2c = oracle.permutation(a, b)
+        H = oracle.string_substring(r2c, m, n)
+        for i in range(n + 1):
+            for j in range(i, n + 1):
+                assert H[i, j] == oracle.llcs_dp(a, b[i:j])
+
+
+def test_dp_hand_cases():
+    assert oracle.llcs_dp([1, 2, 3, 4], [2, 4]) == 2
+    assert oracle.llcs_dp([1, 2, 3], [3, 2, 1]) == 1
+    assert oracle.llcs_dp([], [1, 2]) == 0
+    assert oracle.llcs_dp([0, 1, 0, 1, 1], [1, 0, 1, 1, 0]) == 4
+
+
+def test_fixup_quadrants_equal_whole():
+    a, b = rnd(9, 61, 83, 3)
+    m, n = 61, 83
+    ap = a[::-1].copy()
+    h = np.arange(m, dtype=np.int32)
+    v = (m + np.arange(n)).astype(np.int32)
+    # dependency order: rows with high i first (v flows to low i), then left to right
+    oracle.fixup(ap, b, h, v, 20, m - 20, 0, 40)
+    oracle.fixup(ap, b, h, v, 0, 20, 0, 40)
+    oracle.fixup(ap, b, h, v, 0, m, 40, n - 40)
+    ho, vo = oracle.comb(a, b)
+    assert np.array_equal(h, ho) and np.array_equal(v, vo)
+
+
+def _hash_bigint(c2r):
+    R, M = 4294967279, 4294967291
+    res = 0
+    for x in c2r:
+        t = (R * res + int(x)) % (1 << 64)
+        if t >= 1 << 63:
+            t -= 1 << 64
+        # C remainder truncates toward zero
+        res = abs(t) % M * (1 if t >= 0 else -1)
+    return res
+
+
+def test_hash_wraps_like_int64():
+    a, b = rnd(4, 300, 400, 2)
+    h, v = oracle.comb(a, b)
+    c2r = oracle.from_strands(h, v)[1]
+    assert oracle.perm_hash(c2r) == _hash_bigint(c2r)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (needs the reference tree)")
+def test_from_strands_and_hash_match_reference_build():
+    for seed in range(20):
+        rng = np.random.default_rng(seed)
+        m, n = rng.integers(0, 400), rng.integers(0, 400)
+        a, b = rnd(seed, m, n, 2)
+        h, v = oracle.comb(a, b)
+        r2c, c2r = oracle.from_strands(h, v)
+        rr, rc, hv = oracle.ref_from_strands_hash(h, v)
+        assert np.array_equal(rr, r2c) and np.array_equal(rc, c2r) and hv == oracle.perm_hash(c2r)
+
+
+def test_composition_law_both_glues():
+    rng = np.random.default_rng(77)
+    for t in range(150):
+        s = rng.integers(1, 5)
+        m, n1, n2 = rng.integers(1, 25), rng.integers(0, 25), rng.integers(0, 25)
+        a, b1 = rnd(t, m, n1, s)
+        _, b2 = rnd(t + 1000, 0, n2, s)
+        P = oracle.permutation(a, np.concatenate([b1, b2]))
+        R = oracle.staggered_multiply(oracle.permutation(a, b1), oracle.permutation(a, b2), m, False)
+        assert np.array_equal(P, R)
+        m1, m2, n = rng.integers(0, 25), rng.integers(0, 25), rng.integers(1, 25)
+        a1, b = rnd(t + 2000, m1, n, s)
+        a2, _ = rnd(t + 3000, m2, 0, s)
+        P = oracle.permutation(np.concatenate([a1, a2]), b)
+        R = oracle.staggered_multiply(oracle.permutation(a1, b), oracle.permutation(a2, b), n, True)
+        assert np.array_equal(P, R)
+
+
+def test_steady_ant_equals_distance_product_definition():
+    rng = np.random.default_rng(5)
+    for t in range(300):
+        k = int(rng.integers(1, 10))
+        p = rng.permutation(k).astype(np.int32)
+        q = rng.permutation(k).astype(np.int32)
+        assert np.array_equal(oracle.steady_ant(p, q), oracle.sticky_bruteforce(p, q))
+
+
+def test_golden_fixtures():
+    """Committed fixtures (tests/golden/) reproduce."""
+    path = os.path.join(GOLDEN, "comb_fixtures.json")
+    with open(path) as f:
+        fx = json.load(f)
+    for case in fx["cases"]:
+        a = np.array(case["a"], np.int32)
+        b = np.array(case["b"], np.int32)
+        h, v = oracle.comb(a, b)
+        assert list(h) == case["h"] and list(v) == case["v"]
+        assert oracle.perm_hash(oracle.from_strands(h, v)[1]) == case["hash"]
+
+
+def test_golden_large_digests():
+    import hashlib
+    import lcs_amd
+    with open(os.path.join(GOLDEN, "comb_fixtures.json")) as f:
+        fx = json.load(f)
+    for c in fx["large"]:
+        if c["kind"] == "binary":
+            a, b = lcs_amd.gen_binary(c["seed_a"], c["m"]), lcs_amd.gen_binary(c["seed_b"], c["n"])
+        else:
+            a, b = lcs_amd.gen_uniform(c["seed_a"], c["m"], 4), lcs_amd.gen_uniform(c["seed_b"], c["n"], 4)
+        h, v = oracle.comb(a, b, "mt", threads=4)
+        assert hashlib.sha256(np.concatenate([h, v]).astype("<i4").tobytes()).hexdigest() == c["sha256"]
+        assert oracle.perm_hash(oracle.from_strands(h, v)[1]) == c["hash"]
