@@ -72,6 +72,12 @@ int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len);
  * the last lcs_comb / lcs_comb_device call (HIP events around the kernels). */
 int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms);
 
+/* Diagnostics (no counterpart in the reference): record the start and end
+ * time (100 MHz device clock) of each band of the following chain launches.
+ * max_bands > 0 arms tracing, 0 disarms it; with host_out != NULL copies
+ * 2*max_bands words of the last launch's trace out instead. */
+int lcs_debug_band_trace(lcs_ctx *ctx, int32_t max_bands, uint64_t *host_out);
+
 /* ---- single pair --------------------------------------------------------
  * Replaces an LcsProcedure registered in reg2 (src/lcs_interface_internal.hpp
  * :91, src/algorithm_registry.cpp:149-155) such as Lcs_Semi_Tiled_MT

@@ -1,0 +1,31 @@
+// Launch interface of the steady-ant kernels (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Largest padded core (2^L) the workgroup kernel holds in LDS.
+constexpr int kMaxCoreK = 32768;
+
+struct BraidCoreDesc {
+    const int32_t *P;   // k: row -> shared column
+    const int32_t *Q;   // k: shared row -> column
+    int32_t *R;         // k: row -> column of the sticky product
+    int32_t k, L;       // padded size 2^L >= k
+    int32_t *scratch;   // lcsb_core_scratch_words(k) words
+};
+
+struct AssembleArgs {
+    const int32_t *p, *q;
+    int32_t *out;
+    const int32_t *map_row, *map_col, *core;
+    int32_t p_size, q_size, k, row_glue;
+};
+
+size_t lcsb_core_scratch_words(int32_t k);
+int lcsb_core_L(int32_t k);
+int lcsb_launch_core(const BraidCoreDesc *d_descs, int count, int max_K, hipStream_t stream);
+size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k);
+// staggered_multiply<row_glue>(p, q, k) on device pointers; d_desc is one
+// device BraidCoreDesc the call may overwrite.
+int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
+                         int32_t row_glue, int32_t *out, int32_t *ws, BraidCoreDesc *d_desc, hipStream_t stream);

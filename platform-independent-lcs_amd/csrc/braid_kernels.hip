@@ -1,0 +1,422 @@
+// Steady-ant braid multiplication on the GPU (gfx950).
+//
+// Computes staggered_multiply<RowGlue>(p, q, k) of the reference
+// (src/braid_multiplication.hpp:273-406): the sticky-braid (distance) product
+// of two seaweed permutations glued on k strands.  The k x k core product is
+// the steady ant of src/braid_multiplication.hpp:204-270, organised for a
+// workgroup instead of a recursion:
+//
+//  * the core is padded to K = 2^L with fixed points (strands that never meet
+//    another strand do not change the product), so every node of the
+//    recursion at depth d is an aligned segment of K >> d positions;
+//  * top-down, one stable partition per depth orders the points of P by
+//    (node, row) and those of Q by (node, column) -- the reference's
+//    get_vert_slice / get_horz_slice (:5-36) for all nodes of a depth at
+//    once -- and remembers where each point came from;
+//  * bottom-up, each depth lifts the children's products into the node's
+//    coordinates (the reference's inverse_mapping, :191-202) and runs the ant
+//    passage (:123-189) for all nodes at once.  The passage is cut into
+//    chunks of rows: the dominance-sum difference
+//        delta(r, c) = #{hi points: row < r, col < c} - #{lo points: row >= r, col >= c}
+//    is non-decreasing in r and c, so the ant leaves row r at the first
+//    column T(r) where delta >= 0.  One wave finds T and delta at each chunk
+//    boundary with a single pass over the node's columns; then one thread
+//    per chunk walks it with the reference's incremental updates and places
+//    the good points.
+// The union of the children's points and the per-column view live in LDS
+// (16-bit entries with the lo/hi flag in bit 15), so the walks run at LDS
+// latency.  Cores up to K = 32768 take this path.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "braid_kernels.h"
+
+namespace lcsb {
+
+constexpr int kThreads = 1024;
+constexpr int kChunkRows = 32;
+constexpr uint16_t kLoFlag = 0x8000u;
+constexpr uint16_t kIdx = 0x7fffu;
+
+// Exclusive prefix of the 0/1 flags of keys[0..K) (bit `bit`), written to
+// S[0..K] (S[K] = total).  Each thread owns a contiguous run of elements.
+__device__ void block_scan_bits(const int32_t *keys, int K, int bit, int32_t *S, int32_t *lds_sums)
+{
+    const int t = threadIdx.x;
+    const int per = (K + kThreads - 1) / kThreads;
+    const int lo = min(t * per, K), hi = min(lo + per, K);
+    int cnt = 0;
+    for (int x = lo; x < hi; ++x) cnt += (keys[x] >> bit) & 1;
+    lds_sums[t] = cnt;
+    __syncthreads();
+    // Hillis-Steele over the 1024 per-thread sums
+    for (int off = 1; off < kThreads; off <<= 1) {
+        int v = t >= off ? lds_sums[t - off] : 0;
+        __syncthreads();
+        lds_sums[t] += v;
+        __syncthreads();
+    }
+    int run = lds_sums[t] - cnt;  // exclusive
+    for (int x = lo; x < hi; ++x) {
+        S[x] = run;
+        run += (keys[x] >> bit) & 1;
+    }
+    if (t == kThreads - 1) S[K] = lds_sums[kThreads - 1];
+    __syncthreads();
+}
+
+// One depth of the top-down split: stable partition of every aligned segment
+// of size s by bit `bit` of the key (zeros first).  par[pos] = old position.
+__device__ void split_depth(const int32_t *key, int32_t *key_new, int32_t *par, int K, int s, int bit, int32_t *S,
+                            int32_t *lds_sums)
+{
+    block_scan_bits(key, K, bit, S, lds_sums);
+    for (int x = threadIdx.x; x < K; x += kThreads) {
+        const int seg = x & ~(s - 1);
+        const int b = (key[x] >> bit) & 1;
+        const int ones_before = S[x] - S[seg];
+        const int zeros_total = s - (S[seg + s] - S[seg]);
+        const int pos = b ? seg + zeros_total + ones_before : seg + (x - seg - ones_before);
+        key_new[pos] = key[x];
+        par[pos] = x;
+    }
+    __syncthreads();
+}
+
+// The sticky product of the k-permutations P (row -> shared column) and Q
+// (shared row -> column) for every descriptor; one workgroup per product.
+__global__ __launch_bounds__(kThreads) void braid_core_kernel(const BraidCoreDesc *descs)
+{
+    const BraidCoreDesc D = descs[blockIdx.x];
+    const int k = D.k, L = D.L, K = 1 << L;
+    extern __shared__ uint16_t lds[];
+    uint16_t *U = lds;            // node-local row -> node-local col | lo flag
+    uint16_t *CI = lds + K;       // node-local col -> node-local row | lo flag
+    __shared__ int32_t sums[kThreads];
+    int32_t *keyP = D.scratch, *keyQ = keyP + K, *keyP2 = keyQ + K, *keyQ2 = keyP2 + K;
+    int32_t *S = keyQ2 + K;               // K+1
+    int32_t *R = S + K + 1, *Rn = R + K;  // per depth products
+    int32_t *bT = Rn + K, *bD = bT + K;   // per chunk entry column / delta
+    int32_t *parP = bD + K;               // L * K
+    int32_t *parQ = parP + (size_t)L * K; // L * K
+
+    // depth 0: P ordered by row (key = shared column), Q by column (key = shared row)
+    for (int x = threadIdx.x; x < K; x += kThreads) {
+        keyP[x] = x < k ? D.P[x] : x;
+        if (x >= k) keyQ[x] = x;
+    }
+    for (int r = threadIdx.x; r < k; r += kThreads) keyQ[D.Q[r]] = r;
+    __syncthreads();
+    for (int d = 0; d < L; ++d) {
+        const int s = K >> d, bit = L - d - 1;
+        split_depth(keyP, keyP2, parP + (size_t)d * K, K, s, bit, S, sums);
+        split_depth(keyQ, keyQ2, parQ + (size_t)d * K, K, s, bit, S, sums);
+        int32_t *t0 = keyP; keyP = keyP2; keyP2 = t0;
+        int32_t *t1 = keyQ; keyQ = keyQ2; keyQ2 = t1;
+    }
+    for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
+    __syncthreads();
+
+    for (int d = L - 1; d >= 0; --d) {
+        const int s = K >> d, h = s >> 1;
+        const int32_t *pp = parP + (size_t)d * K, *pq = parQ + (size_t)d * K;  // depth d+1 -> d
+        // lift the children's products into node coordinates
+        for (int x = threadIdx.x; x < K; x += kThreads) {
+            const int p_pos = pp[x];
+            const int cseg = x & ~(h - 1);
+            const int q_pos = pq[cseg + R[x]];
+            const int v0 = p_pos & ~(s - 1);
+            const uint16_t lo = (x & h) ? 0 : kLoFlag;
+            const int lr = p_pos - v0, lc = q_pos - v0;
+            U[p_pos] = (uint16_t)lc | lo;
+            CI[v0 + lc] = (uint16_t)lr | lo;
+            Rn[p_pos] = lc;
+        }
+        __syncthreads();
+        // chunk height: 32 rows, or s/128 for big nodes, balancing the
+        // boundary scans (O(s) each) against the walks (O(rows) each)
+        const int rc = s < kChunkRows ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
+        const int nch = s / rc;            // chunks per node
+        const int total = K / rc;          // chunks at this depth
+        // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc)
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int ch = wave; ch < total; ch += kThreads / 64) {
+            const int v0 = (ch / nch) * s, j = ch % nch;
+            const int Dr = s - j * rc;     // rows below are done; the ant enters row Dr-1
+            int T = 0, delta = 0;
+            if (j > 0) {
+                // T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
+                // z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr, Y = #{lo rows >= Dr}
+                const int per = (s + 63) / 64;
+                const int c_lo = min(lane * per, s), c_hi = min(c_lo + per, s);
+                int zc = 0, yc = 0;
+                for (int c = c_lo; c < c_hi; ++c) {
+                    const uint16_t e = CI[v0 + c];
+                    const int lo_pt = (e & kLoFlag) != 0, row = e & kIdx;
+                    zc += lo_pt ? (row >= Dr) : (row < Dr);
+                    yc += lo_pt && row >= Dr;
+                }
+                // wave-wide sums
+                int Y = yc;
+                for (int off = 32; off > 0; off >>= 1) Y += __shfl_xor(Y, off);
+                int incl = zc;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int v = __shfl_up(incl, off);
+                    if (lane >= off) incl += v;
+                }
+                const int excl = incl - zc;
+                if (Y > 0) {
+                    // the lane whose run holds the Y-th z column finds it
+                    const bool mine = excl < Y && incl >= Y;
+                    int pos = -1;
+                    if (mine) {
+                        int cnt = excl;
+                        for (int c = c_lo; c < c_hi; ++c) {
+                            const uint16_t e = CI[v0 + c];
+                            cnt += (e & kLoFlag) ? ((e & kIdx) >= Dr) : ((e & kIdx) < Dr);
+                            if (cnt == Y) { pos = c; break; }
+                        }
+                    }
+                    const unsigned long long who = __ballot(mine);
+                    const int src = __ffsll((long long)who) - 1;
+                    pos = __shfl(pos, src);
+                    T = pos + 1;
+                } else {
+                    T = 0;
+                }
+                delta = 0;  // delta(Dr, T) = Z(T) - Y = 0 by the choice of T
+            }
+            if (lane == 0) {
+                // enter row Dr-1: delta(Dr-1, T) = delta(Dr, T) - [row Dr-1 counted at T]
+                const uint16_t e = U[v0 + Dr - 1];
+                const int col = e & kIdx;
+                delta -= (e & kLoFlag) ? (col >= T) : (col < T);
+                bT[ch] = T;
+                bD[ch] = delta;
+            }
+        }
+        __syncthreads();
+        for (int ch = threadIdx.x; ch < total; ch += kThreads) {
+            const int v0 = (ch / nch) * s, j = ch % nch;
+            const int r_hi = s - j * rc, r_lo = r_hi - rc;
+            int dr = r_hi - 1, dc = bT[ch], delta = bD[ch];
+            while (dr >= r_lo) {
+                if (dc > 0) {
+                    const uint16_t el = CI[v0 + dc - 1];
+                    const int d_left = delta - ((el & kLoFlag) ? ((el & kIdx) >= dr) : ((el & kIdx) < dr));
+                    const uint16_t ed = U[v0 + dr];
+                    const int d_down = delta + ((ed & kLoFlag) ? ((ed & kIdx) >= dc) : ((ed & kIdx) < dc));
+                    if (d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
+                }
+                // delta(dr, s) >= 0 always; the dc < s test only bounds the walk
+                if (delta < 0 && dc < s) {
+                    const uint16_t ec = CI[v0 + dc];
+                    delta += (ec & kLoFlag) ? ((ec & kIdx) >= dr) : ((ec & kIdx) < dr);
+                    ++dc;
+                } else {
+                    if (dr > 0) {
+                        const uint16_t er = U[v0 + dr - 1];
+                        delta -= (er & kLoFlag) ? ((er & kIdx) >= dc) : ((er & kIdx) < dc);
+                    }
+                    --dr;
+                }
+            }
+        }
+        __syncthreads();
+        int32_t *tmp = R; R = Rn; Rn = tmp;
+    }
+    for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
+}
+
+// ---------------------------------------------------------------- slicing
+// Device-wide stable compaction in three kernels: per-block counts, a scan
+// of the counts in one block, per-block scatter.
+constexpr int kScanBlock = 256, kScanPer = 8, kScanTile = kScanBlock * kScanPer;
+
+struct SliceSpec {
+    // selects element i of `src` (size n) when lo <= key(i) < hi, where
+    // key(i) = src[i]; records out_map[j] = i and out_val[key] or out_val[j]
+    const int32_t *src;
+    int32_t n, lo, hi;
+};
+
+__device__ __forceinline__ bool sel(const SliceSpec &S, int i) { const int v = S.src[i]; return v >= S.lo && v < S.hi; }
+
+__global__ void slice_count_kernel(SliceSpec S, int32_t *block_counts)
+{
+    __shared__ int32_t red[kScanBlock];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    int c = 0;
+    for (int q = 0; q < kScanPer; ++q) c += (base + q < S.n) && sel(S, base + q);
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = kScanBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = red[0];
+}
+
+__global__ void scan_counts_kernel(int32_t *counts, int nblocks)
+{
+    // exclusive scan of counts[0..nblocks) in place, one block of 1024
+    __shared__ int32_t s[1024];
+    const int t = threadIdx.x;
+    const int per = (nblocks + 1023) / 1024;
+    const int lo = min(t * per, nblocks), hi = min(lo + per, nblocks);
+    int run = 0;
+    for (int i = lo; i < hi; ++i) run += counts[i];
+    s[t] = run;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int acc = s[t] - run;
+    for (int i = lo; i < hi; ++i) { const int c = counts[i]; counts[i] = acc; acc += c; }
+}
+
+// mode 0 (vertical slice of p, rows in order):   map[j] = i, val[j] = src[i] - lo
+// mode 1 (horizontal slice of q, cols in order): map[j] = i (a column), val[src[i] - lo] = j
+__global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, int32_t *map, int32_t *val, int mode)
+{
+    __shared__ int32_t s[kScanBlock];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    int c = 0;
+    for (int q = 0; q < kScanPer; ++q) c += (base + q < S.n) && sel(S, base + q);
+    s[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        int v = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int j = block_offsets[blockIdx.x] + s[threadIdx.x] - c;
+    for (int q = 0; q < kScanPer; ++q) {
+        const int i = base + q;
+        if (i < S.n && sel(S, i)) {
+            map[j] = i;
+            if (mode == 0) val[j] = S.src[i] - S.lo;
+            else val[S.src[i] - S.lo] = j;
+            ++j;
+        }
+    }
+}
+
+__global__ void invert_kernel(const int32_t *r2c, int32_t n, int32_t *c2r)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c2r[r2c[i]] = i;
+}
+
+// Writes the product (row -> col) from the core result and the untouched
+// points of p and q (braid_multiplication.hpp:353-401).
+__global__ void assemble_kernel(AssembleArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = A.k, ps = A.p_size, qs = A.q_size;
+    if (i < k) {
+        const int row = A.map_row[i], col = A.map_col[A.core[i]];
+        if (A.row_glue) A.out[qs - k + row] = col;
+        else A.out[row] = col + ps - k;
+    }
+    if (i < ps) {
+        const int c = A.p[i];
+        if (A.row_glue) { if (c >= k) A.out[i + qs - k] = c + qs - k; }
+        else { if (c < ps - k) A.out[i] = c; }
+    }
+    if (i < qs) {
+        const int c = A.q[i];
+        if (A.row_glue) { if (i < qs - k) A.out[i] = c; }
+        else { if (i >= k) A.out[i - k + ps] = c - k + ps; }
+    }
+}
+
+}  // namespace lcsb
+
+using namespace lcsb;
+
+size_t lcsb_core_scratch_words(int32_t k)
+{
+    int L = 0;
+    while ((1 << L) < k) ++L;
+    const size_t K = (size_t)1 << L;
+    return (size_t)9 * K + 1 + (size_t)2 * L * K;
+}
+
+int lcsb_core_L(int32_t k)
+{
+    int L = 0;
+    while ((1 << L) < k) ++L;
+    return L;
+}
+
+int lcsb_launch_core(const BraidCoreDesc *d_descs, int count, int max_K, hipStream_t stream)
+{
+    if (max_K > kMaxCoreK) return -34;
+    const size_t lds = (size_t)2 * max_K * sizeof(uint16_t);
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)braid_core_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(2 * kMaxCoreK * sizeof(uint16_t))) != hipSuccess)
+            return -5;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(braid_core_kernel, dim3(count), dim3(kThreads), lds, stream, d_descs);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// staggered_multiply on device pointers; ws must hold lcsb_multiply_ws_words(...)
+int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
+                         int32_t row_glue, int32_t *out, int32_t *ws, BraidCoreDesc *d_desc, hipStream_t stream)
+{
+    const int nmax = p_size > q_size ? p_size : q_size;
+    int32_t *q_c2r = ws;                  // q_size
+    int32_t *map_row = q_c2r + q_size;    // k
+    int32_t *map_col = map_row + k;       // k
+    int32_t *Pred = map_col + k;          // k
+    int32_t *Qred = Pred + k;             // k
+    int32_t *core = Qred + k;             // k
+    int32_t *counts = core + k;           // nblocks
+    const int nblocks_max = (nmax + kScanTile - 1) / kScanTile + 1;
+    int32_t *core_ws = counts + nblocks_max;
+    if (k > 0) {
+        hipLaunchKernelGGL(invert_kernel, dim3((q_size + 255) / 256), dim3(256), 0, stream, q, q_size, q_c2r);
+        // vertical slice of p: rows whose column lies in the glued range
+        SliceSpec sp{p, p_size, row_glue ? 0 : p_size - k, row_glue ? k : p_size};
+        int nb = (p_size + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
+        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts, map_row, Pred, 0);
+        // horizontal slice of q: columns whose row lies in the glued range
+        SliceSpec sq{q_c2r, q_size, row_glue ? q_size - k : 0, row_glue ? q_size : k};
+        nb = (q_size + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
+        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts, map_col, Qred, 1);
+        BraidCoreDesc h{};
+        h.P = Pred;
+        h.Q = Qred;
+        h.R = core;
+        h.k = k;
+        h.L = lcsb_core_L(k);
+        h.scratch = core_ws;
+        if (hipMemcpyAsync(d_desc, &h, sizeof(h), hipMemcpyHostToDevice, stream) != hipSuccess) return -5;
+        int rc = lcsb_launch_core(d_desc, 1, 1 << h.L, stream);
+        if (rc) return rc;
+    }
+    AssembleArgs A{p, q, out, map_row, map_col, core, p_size, q_size, k, row_glue};
+    const int n = nmax > k ? nmax : k;
+    hipLaunchKernelGGL(assemble_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k)
+{
+    const int nmax = p_size > q_size ? p_size : q_size;
+    const size_t nblocks_max = (size_t)(nmax + kScanTile - 1) / kScanTile + 1;
+    return (size_t)q_size + 5 * (size_t)k + nblocks_max + lcsb_core_scratch_words(k > 0 ? k : 1);
+}

@@ -14,10 +14,13 @@ struct ChainArgs {
     int32_t *v_out;              // n strands
     const int32_t *h_in;         // nullable: initial h strands (reference order), else i
     const int32_t *v_in;         // nullable: initial v strands, else m + j
-    unsigned long long *E;       // 2*n ring words (tag << 32 | strand)
+    unsigned long long *E;       // 2*n ring words (tag << 32 | strand) + 64 trash words
     unsigned *ticket;            // ctrl[0]
     unsigned *abort_flag;        // ctrl[1]
     unsigned long long timeout_ticks;  // 100 MHz realtime ticks per wait
+    unsigned long long *trace;   // nullable: [2g] start, [2g+1] end of band g (realtime ticks)
+    int32_t trace_cap;           // bands the trace buffer holds
+    int32_t pad1;
 };
 
 int lcsk_launch_chain(const ChainArgs &args, int tile_m, int grid_blocks, int block_threads, hipStream_t stream);

@@ -142,11 +142,17 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
     }
 }
 
+// Every band runs the same sequence of memory operations: it reads its top
+// edge from ring slot g&1 (tag g) and writes its bottom edge to slot (g+1)&1
+// (tag g+1).  The prep kernel seeds slot 0 with the grid's top edge (tag 0)
+// and the extract kernel reads v_out from the last band's slot, so no
+// branch on the band's position sits between a load and its wait.
 template <int TM, int C, bool ROWMASK>
 __device__ bool band_pass(const ChainArgs &A, int g, int lane)
 {
     const int m = A.m, n = A.n;
     const int r0 = g * (kWave * TM) + lane * TM;
+    if (A.trace && g < A.trace_cap && lane == 0) A.trace[2 * (size_t)g] = __builtin_amdgcn_s_memrealtime();
     int32_t a[TM];
     uint32_t h[TM];
     bool valid[TM];
@@ -157,78 +163,85 @@ __device__ bool band_pass(const ChainArgs &A, int g, int lane)
         a[k] = valid[k] ? A.a[r] : 0;
         h[k] = valid[k] ? (A.h_in ? (uint32_t)A.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
     }
-    const bool first = (g == 0);
-    const bool last = (g == A.nbands - 1);
     const uint32_t tag_in = (uint32_t)g;
     const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
     const unsigned long long *Ein = A.E + (size_t)(g & 1) * (size_t)n;
     unsigned long long *Eout = A.E + (size_t)((g + 1) & 1) * (size_t)n;
     const int nchunks = (n + kWave + C - 1) / C;
 
-    // prefetch chunk 0
-    int32_t pB = 0;
-    unsigned long long pE = 0;
-    {
-        const int c = lane;
+    // prefetch chunk 0.  Every lane loads from a clamped in-bounds address
+    // (no exec masking, no select), so the waits before each chunk cover
+    // exactly these loads.
+    const int nm1 = n - 1;
+    int32_t pB = A.b[min(lane, nm1)];
+    unsigned long long pE = ld_relaxed(Ein + min(lane, nm1));
+    // Publishes the C bottom-edge strands caught in O during the chunk that
+    // started at step t: O lanes [64-C, 64) hold columns t-64 .. t-64+C-1.
+    // Every lane stores every time (lanes without a column store to a trash
+    // word past the ring), so no path skips the store and the waits for the
+    // prefetch loads can leave it in flight.
+    unsigned long long *trash = A.E + 2 * (size_t)n + lane;
+    auto publish = [&](uint32_t Ox, int t) {
+        const int col = t + C - 2 * kWave + lane;
+        const bool mine = lane >= kWave - C && col >= 0 && col < n;
+        st_relaxed(mine ? Eout + col : trash, tag_out | (unsigned long long)Ox);
+    };
+    // Turns the prefetched ring words of the chunk starting at t into lane 0's
+    // conveyor I (waiting, bounded in time, until they carry this band's tag).
+    auto take_inputs = [&](int t, uint32_t &I) -> bool {
+        const int c = t + lane;
         const bool need = lane < C && c < n;
-        if (need) pB = A.b[c];
-        if (!first && need) pE = ld_relaxed(Ein + c);
-    }
-    uint32_t v = 0, O = 0;
-    int32_t bq = 0;
+        unsigned long long e = pE;
+        bool ok = !need || (uint32_t)(e >> 32) == tag_in;
+        if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+            // The band above is not that far yet: poll.
+            const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+            do {
+                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
+                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                    raise_abort(A.abort_flag);
+                    return false;
+                }
+                if (!ok) e = ld_relaxed(Ein + min(c, nm1));
+                ok = !need || (uint32_t)(e >> 32) == tag_in;
+            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
+        }
+        I = (uint32_t)e;
+        return true;
+    };
+    uint32_t v = 0, O = 0, Osave = 0, I = 0;
+    int32_t bq = 0, IB = pB;
+    if (!take_inputs(0, I)) return false;
     for (int k = 0; k < nchunks; ++k) {
         const int t0 = k * C;
-        uint32_t I;
-        const int32_t IB = pB;
-        {
-            const int c = t0 + lane;
-            const bool need = lane < C && c < n;
-            if (first) {
-                I = A.v_in ? (need ? (uint32_t)A.v_in[c] : 0u) : (uint32_t)(m + c);
-            } else {
-                unsigned long long e = pE;
-                unsigned long long t_start = 0;
-                for (;;) {
-                    const bool ok = !need || (uint32_t)(e >> 32) == tag_in;
-                    if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
-                    if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
-                    const unsigned long long now = __builtin_amdgcn_s_memrealtime();
-                    if (t_start == 0) t_start = now;
-                    else if (now - t_start > A.timeout_ticks) { raise_abort(A.abort_flag); return false; }
-                    __builtin_amdgcn_s_sleep(2);
-                    if (!ok) e = ld_relaxed(Ein + c);
-                }
-                I = (uint32_t)e;
-            }
+        {   // prefetch chunk k+1, then publish chunk k-1: the wait for these
+            // loads at the end of the body then leaves the store in flight
+            const int c = min(t0 + C + lane, nm1);
+            pB = A.b[c];
+            pE = ld_relaxed(Ein + c);
         }
-        {   // prefetch chunk k+1
-            const int c = t0 + C + lane;
-            const bool need = lane < C && c < n;
-            pB = need ? A.b[c] : 0;
-            if (!first) pE = need ? ld_relaxed(Ein + c) : 0ull;
-        }
+        publish(Osave, t0 - C);
         if (t0 >= kWave - 1 && t0 + C <= n)
             band_steps<TM, C, ROWMASK, false>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
         else
             band_steps<TM, C, ROWMASK, true>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
-        // O lanes [64-C, 64) hold the bottom-edge strands of columns
-        // t0-64 .. t0-64+C-1 (captured one step late).
-        const int col = t0 + C - 2 * kWave + lane;
-        if (lane >= kWave - C && col >= 0 && col < n) {
-            if (last) A.v_out[col] = (int32_t)O;
-            else st_relaxed(Eout + col, tag_out | (unsigned long long)O);
+        Osave = O;
+        if (k + 1 < nchunks) {
+            IB = pB;
+            if (!take_inputs(t0 + C, I)) return false;
         }
     }
+    publish(Osave, (nchunks - 1) * C);
 #pragma unroll
     for (int k = 0; k < TM; ++k)
         if (valid[k]) A.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];
+    if (A.trace && g < A.trace_cap && lane == 0) A.trace[2 * (size_t)g + 1] = __builtin_amdgcn_s_memrealtime();
     return true;
 }
 
-template <int TM>
+template <int TM, int C>
 __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
 {
-    constexpr int C = 16;
     const int lane = threadIdx.x & (kWave - 1);
     for (;;) {
         unsigned t = 0;
@@ -243,13 +256,27 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
     }
 }
 
-// Clears the ring tags and the ticket/abort words before a chain launch (a
-// kernel, not hipMemsetAsync, so the pair can live inside a hipGraph).
-__global__ void comb_prep_kernel(unsigned long long *E, size_t words, unsigned *ctrl, int nctrl)
+// Seeds the ring before a chain launch: slot 0 holds the grid's top edge
+// (v_in[c], or the ids m+c) under tag 0 for band 0, slot 1 holds a tag no
+// band waits for; also clears the ticket and abort words.  A kernel, not
+// hipMemsetAsync, so the launch sequence can live inside a hipGraph.
+__global__ void comb_prep_kernel(unsigned long long *E, int32_t n, int32_t m, const int32_t *v_in, unsigned *ctrl,
+                                 int nctrl)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) E[i] = 0ull;
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+        const uint32_t v0 = v_in ? (uint32_t)v_in[c] : (uint32_t)(m + c);
+        E[c] = (unsigned long long)v0;                       // tag 0
+        E[(size_t)n + c] = 0xffffffff00000000ull;            // no band has this tag
+    }
     if (blockIdx.x == 0 && (int)threadIdx.x < nctrl) ctrl[threadIdx.x] = 0u;
+}
+
+// v_out = the strands the last band left in its slot.
+__global__ void comb_extract_kernel(const unsigned long long *E, int32_t n, int32_t *v_out)
+{
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) v_out[c] = (int32_t)(uint32_t)E[c];
 }
 
 // Identity strands for empty grids (m == 0 or n == 0).
@@ -264,22 +291,30 @@ __global__ void comb_identity_kernel(int32_t *h_out, int32_t m, int32_t *v_out, 
 
 using namespace lcsk;
 
+template <int TM, int C>
+static void launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_threads, hipStream_t stream)
+{
+    comb_chain_kernel<TM, C><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+}
+
 int lcsk_launch_chain(const ChainArgs &args, int tile_m, int grid_blocks, int block_threads, hipStream_t stream)
 {
-    const size_t words = 2ull * (size_t)args.n;
-    int prep_blocks = (int)((words + 255) / 256);
-    if (prep_blocks < 1) prep_blocks = 1;
-    if (prep_blocks > 2048) prep_blocks = 2048;
-    hipLaunchKernelGGL(comb_prep_kernel, dim3(prep_blocks), dim3(256), 0, stream, args.E, words, args.ticket, 2);
+    int fill_blocks = (args.n + 255) / 256;
+    if (fill_blocks < 1) fill_blocks = 1;
+    if (fill_blocks > 2048) fill_blocks = 2048;
+    hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks), dim3(256), 0, stream, args.E, args.n, args.m, args.v_in,
+                       args.ticket, 2);
     switch (tile_m) {
-    case 1: hipLaunchKernelGGL(comb_chain_kernel<1>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
-    case 2: hipLaunchKernelGGL(comb_chain_kernel<2>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
-    case 4: hipLaunchKernelGGL(comb_chain_kernel<4>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
-    case 8: hipLaunchKernelGGL(comb_chain_kernel<8>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
-    case 16: hipLaunchKernelGGL(comb_chain_kernel<16>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
-    case 32: hipLaunchKernelGGL(comb_chain_kernel<32>, dim3(grid_blocks), dim3(block_threads), 0, stream, args); break;
+    case 1: launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
+    case 2: launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
+    case 4: launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
+    case 8: launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
+    case 16: launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
+    case 32: launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
     default: return -22;
     }
+    const unsigned long long *last_slot = args.E + (size_t)(args.nbands & 1) * (size_t)args.n;
+    hipLaunchKernelGGL(comb_extract_kernel, dim3(fill_blocks), dim3(256), 0, stream, last_slot, args.n, args.v_out);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -82,7 +82,7 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     if (wps < 1 || wps > 8) return LCS_EINVAL;
     const long long rows_per_band = 64LL * tm;
     const long long nbands = ((long long)m + rows_per_band - 1) / rows_per_band;
-    int rc = ctx->ring.reserve(ctx, 2ull * (size_t)n * sizeof(unsigned long long));
+    int rc = ctx->ring.reserve(ctx, (2ull * (size_t)n + 64) * sizeof(unsigned long long));
     if (rc) return rc;
     ChainArgs A{};
     A.a = d_a;
@@ -98,6 +98,8 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     A.ticket = ctx->ctrl;
     A.abort_flag = ctx->ctrl + 1;
     A.timeout_ticks = ctx->timeout_ticks;
+    A.trace = ctx->band_trace;
+    A.trace_cap = ctx->band_trace_cap;
     const int waves_per_block = 4;
     long long waves = std::min<long long>(nbands, 4LL * ctx->num_cus * wps);
     int blocks = (int)((waves + waves_per_block - 1) / waves_per_block);
@@ -164,6 +166,7 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     ctx->out_h.release(ctx);
     ctx->out_v.release(ctx);
     if (ctx->ctrl) (void)hipFree(ctx->ctrl);
+    if (ctx->band_trace) (void)hipFree(ctx->band_trace);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -238,6 +241,31 @@ int lcs_comb_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *
     if (rc) return rc;
     if (hipEventRecord(ctx->ev1, s) != hipSuccess) return LCS_EIO;
     ctx->pending_timing = true;
+    return LCS_OK;
+}
+
+int lcs_debug_band_trace(lcs_ctx *ctx, int32_t max_bands, uint64_t *host_out)
+{
+    // max_bands > 0: (re)arm tracing of up to max_bands bands; 0: disarm;
+    // host_out != NULL: copy the last trace (2 words per band) out.
+    if (!ctx || max_bands < 0) return LCS_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    if (host_out) {
+        if (!ctx->band_trace || max_bands > ctx->band_trace_cap) return LCS_EINVAL;
+        if (hipMemcpy(host_out, ctx->band_trace, (size_t)max_bands * 16, hipMemcpyDeviceToHost) != hipSuccess)
+            return LCS_EIO;
+        return LCS_OK;
+    }
+    if (ctx->band_trace) {
+        (void)hipFree(ctx->band_trace);
+        ctx->band_trace = nullptr;
+        ctx->band_trace_cap = 0;
+    }
+    if (max_bands == 0) return LCS_OK;
+    void *p = nullptr;
+    if (hipMalloc(&p, (size_t)max_bands * 16) != hipSuccess) return LCS_ENOMEM;
+    ctx->band_trace = (unsigned long long *)p;
+    ctx->band_trace_cap = max_bands;
     return LCS_OK;
 }
 

@@ -34,6 +34,8 @@ struct lcs_ctx {
     bool pending_timing = false;
     lcs::DevBuf ring, in_a, in_b, out_h, out_v;
     lcs::Extras *extras = nullptr;
+    unsigned long long *band_trace = nullptr;  // device buffer set by lcs_debug_band_trace
+    int32_t band_trace_cap = 0;
 };
 
 namespace lcs {

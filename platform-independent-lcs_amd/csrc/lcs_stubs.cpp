@@ -6,11 +6,6 @@ namespace lcs {
 void destroy_extras(lcs_ctx *ctx) { (void)ctx; }
 }
 
-extern "C" int lcs_braid_multiply(lcs_ctx *, const int32_t *, int32_t, const int32_t *, int32_t, int32_t, int32_t,
-                                  int32_t *)
-{
-    return LCS_EIO;
-}
 
 extern "C" int lcs_batch_comb(lcs_ctx *, int32_t, const int32_t *, const int64_t *, const int32_t *,
                               const int32_t *, const int64_t *, const int32_t *, const lcs_schedule *, int32_t,
