@@ -134,6 +134,31 @@ int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs,
                    const lcs_schedule *scheds, int32_t use_graph,
                    int32_t *h_out, int32_t *v_out);
 
+/* ---- learned schedule cost model ---------------------------------------
+ * No counterpart in the reference (its variant choice is a compile-time
+ * name, src/algorithm_registry.cpp:127-142).  An fp32 MLP
+ * (n_in -> hidden -> hidden -> n_out, ReLU) trained on the GPU by the
+ * library's own kernels on the timings the autotuner records; it predicts a
+ * cost per candidate schedule from a pair's features and the cheapest
+ * prediction is chosen.  Schedules only change speed, never results.
+ * X is n x n_in, T and M (mask of measured entries) are n x n_out,
+ * row-major floats, host pointers. */
+typedef struct lcs_mlp lcs_mlp;
+int lcs_mlp_create(lcs_ctx *ctx, int32_t n_in, int32_t hidden, int32_t n_out, uint64_t seed, lcs_mlp **out);
+int lcs_mlp_destroy(lcs_mlp *mlp);
+int lcs_mlp_param_count(lcs_mlp *mlp, int32_t *count);
+int lcs_mlp_get_params(lcs_mlp *mlp, float *out);
+int lcs_mlp_set_params(lcs_mlp *mlp, const float *in);
+int lcs_mlp_predict(lcs_mlp *mlp, const float *X, int32_t n, float *Y);
+/* masked mean squared error and its gradient (param_count floats) */
+int lcs_mlp_loss_grad(lcs_mlp *mlp, const float *X, const float *T, const float *M, int32_t n, float *loss,
+                      float *grad);
+/* `steps` full-batch Adam steps (lr), loss after the last step */
+int lcs_mlp_train(lcs_mlp *mlp, const float *X, const float *T, const float *M, int32_t n, int32_t steps, float lr,
+                  float *final_loss);
+/* index of the smallest predicted cost per row, computed on the GPU */
+int lcs_mlp_choose(lcs_mlp *mlp, const float *X, int32_t n, int32_t *choice);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,12 @@ struct BraidCoreDesc {
     int32_t *scratch;   // lcsb_core_scratch_words(k) words
 };
 
+// Core descriptors passed by value in the kernel arguments, one per block.
+constexpr int kCoreBatch = 64;
+struct CoreBatch {
+    BraidCoreDesc d[kCoreBatch];
+};
+
 struct AssembleArgs {
     const int32_t *p, *q;
     int32_t *out;
@@ -23,9 +29,9 @@ struct AssembleArgs {
 
 size_t lcsb_core_scratch_words(int32_t k);
 int lcsb_core_L(int32_t k);
-int lcsb_launch_core(const BraidCoreDesc *d_descs, int count, int max_K, hipStream_t stream);
+// One workgroup per descriptor (host array, copied into the launch arguments).
+int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream);
 size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k);
-// staggered_multiply<row_glue>(p, q, k) on device pointers; d_desc is one
-// device BraidCoreDesc the call may overwrite.
+// staggered_multiply<row_glue>(p, q, k) on device pointers.
 int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
-                         int32_t row_glue, int32_t *out, int32_t *ws, BraidCoreDesc *d_desc, hipStream_t stream);
+                         int32_t row_glue, int32_t *out, int32_t *ws, hipStream_t stream);

@@ -85,9 +85,9 @@ __device__ void split_depth(const int32_t *key, int32_t *key_new, int32_t *par, 
 
 // The sticky product of the k-permutations P (row -> shared column) and Q
 // (shared row -> column) for every descriptor; one workgroup per product.
-__global__ __launch_bounds__(kThreads) void braid_core_kernel(const BraidCoreDesc *descs)
+__global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
 {
-    const BraidCoreDesc D = descs[blockIdx.x];
+    const BraidCoreDesc D = batch.d[blockIdx.x];
     const int k = D.k, L = D.L, K = 1 << L;
     extern __shared__ uint16_t lds[];
     uint16_t *U = lds;            // node-local row -> node-local col | lo flag
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kThreads) void braid_core_kernel(const BraidCoreDes
         keyP[x] = x < k ? D.P[x] : x;
         if (x >= k) keyQ[x] = x;
     }
-    for (int r = threadIdx.x; r < k; r += kThreads) keyQ[D.Q[r]] = r;
+    for (int r = threadIdx.x; r < k; r += kThreads)
+        if ((uint32_t)D.Q[r] < (uint32_t)k) keyQ[D.Q[r]] = r;
     __syncthreads();
     for (int d = 0; d < L; ++d) {
         const int s = K >> d, bit = L - d - 1;
@@ -280,7 +281,8 @@ __global__ void scan_counts_kernel(int32_t *counts, int nblocks)
 
 // mode 0 (vertical slice of p, rows in order):   map[j] = i, val[j] = src[i] - lo
 // mode 1 (horizontal slice of q, cols in order): map[j] = i (a column), val[src[i] - lo] = j
-__global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, int32_t *map, int32_t *val, int mode)
+__global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, int32_t *map, int32_t *val, int mode,
+                                     int cap)
 {
     __shared__ int32_t s[kScanBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
@@ -297,7 +299,7 @@ __global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, 
     int j = block_offsets[blockIdx.x] + s[threadIdx.x] - c;
     for (int q = 0; q < kScanPer; ++q) {
         const int i = base + q;
-        if (i < S.n && sel(S, i)) {
+        if (i < S.n && sel(S, i) && j < cap) {
             map[j] = i;
             if (mode == 0) val[j] = S.src[i] - S.lo;
             else val[S.src[i] - S.lo] = j;
@@ -309,7 +311,7 @@ __global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, 
 __global__ void invert_kernel(const int32_t *r2c, int32_t n, int32_t *c2r)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) c2r[r2c[i]] = i;
+    if (i < n && (uint32_t)r2c[i] < (uint32_t)n) c2r[r2c[i]] = i;
 }
 
 // Writes the product (row -> col) from the core result and the untouched
@@ -318,10 +320,13 @@ __global__ void assemble_kernel(AssembleArgs A)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = A.k, ps = A.p_size, qs = A.q_size;
-    if (i < k) {
+    // the range tests only matter after an aborted comb fed undefined input
+    if (i < k && (uint32_t)A.core[i] < (uint32_t)k) {
         const int row = A.map_row[i], col = A.map_col[A.core[i]];
-        if (A.row_glue) A.out[qs - k + row] = col;
-        else A.out[row] = col + ps - k;
+        if ((uint32_t)row < (uint32_t)ps) {
+            if (A.row_glue) A.out[qs - k + row] = col;
+            else A.out[row] = col + ps - k;
+        }
     }
     if (i < ps) {
         const int c = A.p[i];
@@ -354,10 +359,8 @@ int lcsb_core_L(int32_t k)
     return L;
 }
 
-int lcsb_launch_core(const BraidCoreDesc *d_descs, int count, int max_K, hipStream_t stream)
+int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
 {
-    if (max_K > kMaxCoreK) return -34;
-    const size_t lds = (size_t)2 * max_K * sizeof(uint16_t);
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)braid_core_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -365,13 +368,25 @@ int lcsb_launch_core(const BraidCoreDesc *d_descs, int count, int max_K, hipStre
             return -5;
         attr_set = true;
     }
-    hipLaunchKernelGGL(braid_core_kernel, dim3(count), dim3(kThreads), lds, stream, d_descs);
+    // descriptors travel by value in the kernel arguments
+    for (int base = 0; base < count; base += kCoreBatch) {
+        CoreBatch batch{};
+        const int cnt = count - base < kCoreBatch ? count - base : kCoreBatch;
+        int max_K = 1;
+        for (int i = 0; i < cnt; ++i) {
+            batch.d[i] = descs[base + i];
+            if ((1 << batch.d[i].L) > max_K) max_K = 1 << batch.d[i].L;
+        }
+        if (max_K > kMaxCoreK) return -34;
+        const size_t lds = (size_t)2 * max_K * sizeof(uint16_t);
+        hipLaunchKernelGGL(braid_core_kernel, dim3(cnt), dim3(kThreads), lds, stream, batch);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
 // staggered_multiply on device pointers; ws must hold lcsb_multiply_ws_words(...)
 int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
-                         int32_t row_glue, int32_t *out, int32_t *ws, BraidCoreDesc *d_desc, hipStream_t stream)
+                         int32_t row_glue, int32_t *out, int32_t *ws, hipStream_t stream)
 {
     const int nmax = p_size > q_size ? p_size : q_size;
     int32_t *q_c2r = ws;                  // q_size
@@ -390,13 +405,13 @@ int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int
         int nb = (p_size + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts);
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
-        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts, map_row, Pred, 0);
+        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts, map_row, Pred, 0, k);
         // horizontal slice of q: columns whose row lies in the glued range
         SliceSpec sq{q_c2r, q_size, row_glue ? q_size - k : 0, row_glue ? q_size : k};
         nb = (q_size + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts);
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
-        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts, map_col, Qred, 1);
+        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts, map_col, Qred, 1, k);
         BraidCoreDesc h{};
         h.P = Pred;
         h.Q = Qred;
@@ -404,8 +419,7 @@ int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int
         h.k = k;
         h.L = lcsb_core_L(k);
         h.scratch = core_ws;
-        if (hipMemcpyAsync(d_desc, &h, sizeof(h), hipMemcpyHostToDevice, stream) != hipSuccess) return -5;
-        int rc = lcsb_launch_core(d_desc, 1, 1 << h.L, stream);
+        int rc = lcsb_launch_core(&h, 1, stream);
         if (rc) return rc;
     }
     AssembleArgs A{p, q, out, map_row, map_col, core, p_size, q_size, k, row_glue};

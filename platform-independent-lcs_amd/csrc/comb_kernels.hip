@@ -148,11 +148,11 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
 // and the extract kernel reads v_out from the last band's slot, so no
 // branch on the band's position sits between a load and its wait.
 template <int TM, int C, bool ROWMASK>
-__device__ bool band_pass(const ChainArgs &A, int g, int lane)
+__device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane, int t_id)
 {
-    const int m = A.m, n = A.n;
+    const int m = P.m, n = P.n;
     const int r0 = g * (kWave * TM) + lane * TM;
-    if (A.trace && g < A.trace_cap && lane == 0) A.trace[2 * (size_t)g] = __builtin_amdgcn_s_memrealtime();
+    if (A.trace && t_id < A.trace_cap && lane == 0) A.trace[2 * (size_t)t_id] = __builtin_amdgcn_s_memrealtime();
     int32_t a[TM];
     uint32_t h[TM];
     bool valid[TM];
@@ -160,27 +160,27 @@ __device__ bool band_pass(const ChainArgs &A, int g, int lane)
     for (int k = 0; k < TM; ++k) {
         const int r = r0 + k;
         valid[k] = ROWMASK ? (r < m) : true;
-        a[k] = valid[k] ? A.a[r] : 0;
-        h[k] = valid[k] ? (A.h_in ? (uint32_t)A.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
+        a[k] = valid[k] ? P.a[r] : 0;
+        h[k] = valid[k] ? (P.h_in ? (uint32_t)P.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
     }
     const uint32_t tag_in = (uint32_t)g;
     const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
-    const unsigned long long *Ein = A.E + (size_t)(g & 1) * (size_t)n;
-    unsigned long long *Eout = A.E + (size_t)((g + 1) & 1) * (size_t)n;
+    const unsigned long long *Ein = P.E + (size_t)(g & 1) * (size_t)n;
+    unsigned long long *Eout = P.E + (size_t)((g + 1) & 1) * (size_t)n;
     const int nchunks = (n + kWave + C - 1) / C;
 
     // prefetch chunk 0.  Every lane loads from a clamped in-bounds address
     // (no exec masking, no select), so the waits before each chunk cover
     // exactly these loads.
     const int nm1 = n - 1;
-    int32_t pB = A.b[min(lane, nm1)];
+    int32_t pB = P.b[min(lane, nm1)];
     unsigned long long pE = ld_relaxed(Ein + min(lane, nm1));
     // Publishes the C bottom-edge strands caught in O during the chunk that
     // started at step t: O lanes [64-C, 64) hold columns t-64 .. t-64+C-1.
     // Every lane stores every time (lanes without a column store to a trash
     // word past the ring), so no path skips the store and the waits for the
     // prefetch loads can leave it in flight.
-    unsigned long long *trash = A.E + 2 * (size_t)n + lane;
+    unsigned long long *trash = P.E + 2 * (size_t)n + lane;
     auto publish = [&](uint32_t Ox, int t) {
         const int col = t + C - 2 * kWave + lane;
         const bool mine = lane >= kWave - C && col >= 0 && col < n;
@@ -217,7 +217,7 @@ __device__ bool band_pass(const ChainArgs &A, int g, int lane)
         {   // prefetch chunk k+1, then publish chunk k-1: the wait for these
             // loads at the end of the body then leaves the store in flight
             const int c = min(t0 + C + lane, nm1);
-            pB = A.b[c];
+            pB = P.b[c];
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C);
@@ -234,8 +234,8 @@ __device__ bool band_pass(const ChainArgs &A, int g, int lane)
     publish(Osave, (nchunks - 1) * C);
 #pragma unroll
     for (int k = 0; k < TM; ++k)
-        if (valid[k]) A.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];
-    if (A.trace && g < A.trace_cap && lane == 0) A.trace[2 * (size_t)g + 1] = __builtin_amdgcn_s_memrealtime();
+        if (valid[k]) P.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];
+    if (A.trace && t_id < A.trace_cap && lane == 0) A.trace[2 * (size_t)t_id + 1] = __builtin_amdgcn_s_memrealtime();
     return true;
 }
 
@@ -247,36 +247,75 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         unsigned t = 0;
         if (lane == 0) t = atomicAdd(A.ticket, 1u);
         t = __builtin_amdgcn_readfirstlane(t);
-        if ((int)t >= A.nbands) break;
+        if ((int)t >= A.ntasks) break;
         if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) break;
-        const int g = (int)t;
-        const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)A.m;
-        const bool ok = rowmask ? band_pass<TM, C, true>(A, g, lane) : band_pass<TM, C, false>(A, g, lane);
+        const int2 task = A.tasks[t];
+        const PairDesc P = A.pairs[task.x];
+        const int g = task.y;
+        const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
+        const bool ok = rowmask ? band_pass<TM, C, true>(A, P, g, lane, (int)t)
+                                : band_pass<TM, C, false>(A, P, g, lane, (int)t);
         if (!ok) break;
     }
 }
 
-// Seeds the ring before a chain launch: slot 0 holds the grid's top edge
-// (v_in[c], or the ids m+c) under tag 0 for band 0, slot 1 holds a tag no
-// band waits for; also clears the ticket and abort words.  A kernel, not
-// hipMemsetAsync, so the launch sequence can live inside a hipGraph.
-__global__ void comb_prep_kernel(unsigned long long *E, int32_t n, int32_t m, const int32_t *v_in, unsigned *ctrl,
-                                 int nctrl)
+// Seeds the rings before a chain launch (blockIdx.y = pair): slot 0 holds
+// the grid's top edge (v_in[c], or the ids m+c) under tag 0 for band 0,
+// slot 1 holds a tag no band waits for.  A pair with no band (n == 0) gets
+// its h_out here.  Block (0,0) clears the ticket and abort words.  A kernel,
+// not hipMemsetAsync, so the launch sequence can live inside a hipGraph.
+__global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, int nctrl)
 {
+    const PairDesc P = pairs[blockIdx.y];
     const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
-        const uint32_t v0 = v_in ? (uint32_t)v_in[c] : (uint32_t)(m + c);
-        E[c] = (unsigned long long)v0;                       // tag 0
-        E[(size_t)n + c] = 0xffffffff00000000ull;            // no band has this tag
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride) {
+        const uint32_t v0 = P.v_in ? (uint32_t)P.v_in[c] : (uint32_t)(P.m + c);
+        P.E[c] = (unsigned long long)v0;                       // tag 0
+        P.E[(size_t)P.n + c] = 0xffffffff00000000ull;          // no band has this tag
     }
-    if (blockIdx.x == 0 && (int)threadIdx.x < nctrl) ctrl[threadIdx.x] = 0u;
+    if (P.nbands == 0)
+        for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P.m; r += stride)
+            P.h_out[r] = P.h_in ? P.h_in[r] : r;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nctrl) ctrl[threadIdx.x] = 0u;
 }
 
-// v_out = the strands the last band left in its slot.
-__global__ void comb_extract_kernel(const unsigned long long *E, int32_t n, int32_t *v_out)
+// v_out = the strands the last band left in its slot (blockIdx.y = pair).
+__global__ void comb_extract_kernel(const PairDesc *pairs)
 {
+    const PairDesc P = pairs[blockIdx.y];
+    const unsigned long long *slot = P.E + (size_t)(P.nbands & 1) * (size_t)P.n;
     const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) v_out[c] = (int32_t)(uint32_t)E[c];
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride) P.v_out[c] = (int32_t)(uint32_t)slot[c];
+}
+
+// row_to_col of each pair's permutation (blockIdx.y = pair): the strand that
+// leaves row l on the right goes to column n+l, the one leaving column j at
+// the bottom to column j (permutation_matrix.hpp:88-115).
+__global__ void from_strands_kernel(const PairDesc *pairs, int32_t *perm, const int64_t *perm_off)
+{
+    const PairDesc P = pairs[blockIdx.y];
+    int32_t *r2c = perm + perm_off[blockIdx.y];
+    const int stride = gridDim.x * blockDim.x;
+    const uint32_t size = (uint32_t)(P.m + P.n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.m + P.n; i += stride) {
+        // ids are always < m+n after a completed comb; the test keeps an
+        // aborted one (whose strands are undefined) from writing out of range
+        const uint32_t id = (uint32_t)(i < P.m ? P.h_out[i] : P.v_out[i - P.m]);
+        if (id < size) r2c[id] = i < P.m ? P.n + i : i - P.m;
+    }
+}
+
+__global__ void invert_perm_kernel(const int32_t *r2c, int32_t size, int32_t *c2r)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < size && (uint32_t)r2c[i] < (uint32_t)size) c2r[r2c[i]] = i;
+}
+
+__global__ void strands_from_c2r_kernel(const int32_t *c2r, int32_t m, int32_t n, int32_t *h_out, int32_t *v_out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v_out[i] = c2r[i];
+    else if (i < n + m) h_out[i - n] = c2r[i];
 }
 
 // Identity strands for empty grids (m == 0 or n == 0).
@@ -297,24 +336,48 @@ static void launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_th
     comb_chain_kernel<TM, C><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
 }
 
-int lcsk_launch_chain(const ChainArgs &args, int tile_m, int grid_blocks, int block_threads, hipStream_t stream)
+int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs, int max_n, int tile_m,
+                      int grid_blocks, int block_threads, hipStream_t stream)
 {
-    int fill_blocks = (args.n + 255) / 256;
+    if (npairs <= 0) return 0;
+    int fill_blocks = (max_n + 255) / 256;
     if (fill_blocks < 1) fill_blocks = 1;
-    if (fill_blocks > 2048) fill_blocks = 2048;
-    hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks), dim3(256), 0, stream, args.E, args.n, args.m, args.v_in,
-                       args.ticket, 2);
-    switch (tile_m) {
-    case 1: launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
-    case 2: launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
-    case 4: launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
-    case 8: launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
-    case 16: launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
-    case 32: launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
-    default: return -22;
+    if (fill_blocks > 1024) fill_blocks = 1024;
+    hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs, args.ticket, 2);
+    if (args.ntasks > 0) {
+        switch (tile_m) {
+        case 1: launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
+        case 2: launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
+        case 4: launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
+        case 8: launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
+        case 16: launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
+        case 32: launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
+        default: return -22;
+        }
     }
-    const unsigned long long *last_slot = args.E + (size_t)(args.nbands & 1) * (size_t)args.n;
-    hipLaunchKernelGGL(comb_extract_kernel, dim3(fill_blocks), dim3(256), 0, stream, last_slot, args.n, args.v_out);
+    hipLaunchKernelGGL(comb_extract_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, int32_t *perm,
+                             const int64_t *d_perm_off, hipStream_t stream)
+{
+    if (npairs <= 0) return 0;
+    int blocks = (max_len + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(from_strands_kernel, dim3(blocks, npairs), dim3(256), 0, stream, d_pairs, perm, d_perm_off);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_strands_from_perm(const int32_t *r2c, int32_t m, int32_t n, int32_t *scratch, int32_t *h_out,
+                                  int32_t *v_out, hipStream_t stream)
+{
+    const int size = m + n;
+    if (size == 0) return 0;
+    hipLaunchKernelGGL(invert_perm_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, r2c, size, scratch);
+    hipLaunchKernelGGL(strands_from_c2r_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, scratch, m, n, h_out,
+                       v_out);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -37,16 +37,15 @@ extern "C" int lcs_braid_multiply(lcs_ctx *ctx, const int32_t *p_row_to_col, int
     const size_t ws_words = lcsb_multiply_ws_words(p_size, q_size, k);
     const size_t words = (size_t)p_size + q_size + out_size + ws_words + 16;
     DevBuf buf;
-    int rc = buf.reserve(ctx, words * 4 + sizeof(BraidCoreDesc) + 256);
+    int rc = buf.reserve(ctx, words * 4);
     if (rc) return rc;
     int32_t *dp = (int32_t *)buf.ptr, *dq = dp + p_size, *dout = dq + q_size, *dws = dout + out_size;
-    BraidCoreDesc *ddesc = (BraidCoreDesc *)(((uintptr_t)(dws + ws_words) + 255) & ~(uintptr_t)255);
     hipStream_t s = ctx->stream;
     bool ok = (!p_size || hipMemcpyAsync(dp, p_row_to_col, (size_t)p_size * 4, hipMemcpyHostToDevice, s) == hipSuccess) &&
               (!q_size || hipMemcpyAsync(dq, q_row_to_col, (size_t)q_size * 4, hipMemcpyHostToDevice, s) == hipSuccess);
     if (!ok) { buf.release(ctx); return LCS_EIO; }
     if (hipEventRecord(ctx->ev0, s) != hipSuccess) { buf.release(ctx); return LCS_EIO; }
-    rc = lcsb_multiply_device(dp, p_size, dq, q_size, k, row_glue ? 1 : 0, dout, dws, ddesc, s);
+    rc = lcsb_multiply_device(dp, p_size, dq, q_size, k, row_glue ? 1 : 0, dout, dws, s);
     if (!rc) {
         (void)hipEventRecord(ctx->ev1, s);
         ok = hipMemcpyAsync(out_row_to_col, dout, (size_t)out_size * 4, hipMemcpyDeviceToHost, s) == hipSuccess &&

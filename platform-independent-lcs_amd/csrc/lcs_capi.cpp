@@ -9,8 +9,10 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "lcs_hip.h"
+#include "braid_kernels.h"
 #include "comb_kernels.h"
 #include "lcs_internal.h"
 
@@ -61,9 +63,199 @@ int default_tile_m(const lcs_ctx *ctx, int32_t m)
     return tm;
 }
 
-static bool valid_tile_m(int tm) { return tm == 1 || tm == 2 || tm == 4 || tm == 8 || tm == 16 || tm == 32; }
+bool valid_tile_m(int tm) { return tm == 1 || tm == 2 || tm == 4 || tm == 8 || tm == 16 || tm == 32; }
 
-// Enqueue one single-pair chain on `stream` with device pointers.
+int32_t bands_of(int32_t m, int32_t n, int tm)
+{
+    if (m <= 0 || n <= 0) return 0;
+    return (int32_t)(((long long)m + 64LL * tm - 1) / (64LL * tm));
+}
+
+// Stages host bytes into the context's pinned upload buffer and copies them
+// to `dst` on `stream`.  The previous upload is waited for first, so the
+// pinned buffer is never rewritten while a copy from it is pending.
+int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream)
+{
+    if (bytes == 0) return LCS_OK;
+    if (hipEventSynchronize(ctx->upload_done) != hipSuccess) return LCS_EIO;
+    if (bytes > ctx->pinned_bytes) {
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+        void *p = nullptr;
+        if (hipHostMalloc(&p, bytes * 2, hipHostMallocDefault) != hipSuccess) return LCS_ENOMEM;
+        ctx->pinned = p;
+        ctx->pinned_bytes = bytes * 2;
+    }
+    std::memcpy(ctx->pinned, src, bytes);
+    if (hipMemcpyAsync(dst, ctx->pinned, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) return LCS_EIO;
+    if (hipEventRecord(ctx->upload_done, stream) != hipSuccess) return LCS_EIO;
+    return LCS_OK;
+}
+
+// Prepares one chain launch over `pairs` (one tile height): assigns each
+// pair its ring inside `ring` (which must hold pairs_ring_words(pairs)
+// words), lists the tasks band-major (all pairs start at once, and every
+// band comes after the band above it) and uploads descriptors and tasks to
+// `descs`.  Fills `L` for launch_prepared().
+size_t pairs_ring_words(const std::vector<PairDesc> &pairs)
+{
+    size_t w = 0;
+    for (const auto &P : pairs) w += lcsk_ring_words(P.n);
+    return w;
+}
+
+int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
+                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream)
+{
+    if (!valid_tile_m(tm) || wps < 1 || wps > 8) return LCS_EINVAL;
+    const int np = (int)pairs.size();
+    L = PreparedLaunch{};
+    if (np == 0) return LCS_OK;
+    int32_t max_bands = 0, max_n = 0;
+    unsigned long long *E = ring;
+    for (auto &P : pairs) {
+        P.tile_m = tm;
+        P.nbands = bands_of(P.m, P.n, tm);
+        P.E = E;
+        E += lcsk_ring_words(P.n);
+        max_bands = std::max(max_bands, P.nbands);
+        max_n = std::max(max_n, std::max(P.n, P.m));
+    }
+    std::vector<int2> tasks;
+    for (int32_t g = 0; g < max_bands; ++g)
+        for (int p = 0; p < np; ++p)
+            if (g < pairs[(size_t)p].nbands) tasks.push_back(make_int2(p, g));
+    const size_t pair_bytes = sizeof(PairDesc) * (size_t)np, task_bytes = sizeof(int2) * tasks.size();
+    int rc = descs.reserve(ctx, pair_bytes + task_bytes + 256);
+    if (rc) return rc;
+    std::vector<unsigned char> blob(pair_bytes + task_bytes);
+    std::memcpy(blob.data(), pairs.data(), pair_bytes);
+    if (task_bytes) std::memcpy(blob.data() + pair_bytes, tasks.data(), task_bytes);
+    if ((rc = upload(ctx, descs.ptr, blob.data(), blob.size(), stream))) return rc;
+    ChainArgs &A = L.args;
+    A.pairs = (const PairDesc *)descs.ptr;
+    A.tasks = (const int2 *)((unsigned char *)descs.ptr + pair_bytes);
+    A.ntasks = (int32_t)tasks.size();
+    A.ticket = ctrl;
+    A.abort_flag = ctrl + 1;
+    A.timeout_ticks = ctx->timeout_ticks;
+    A.trace = ctx->band_trace;
+    A.trace_cap = ctx->band_trace_cap;
+    const int waves_per_block = 4;
+    const long long waves = std::min<long long>((long long)tasks.size(), 4LL * ctx->num_cus * wps);
+    L.blocks = (int)std::max<long long>(1, (waves + waves_per_block - 1) / waves_per_block);
+    L.threads = 64 * waves_per_block;
+    L.npairs = np;
+    L.max_n = max_n;
+    L.tile_m = tm;
+    return LCS_OK;
+}
+
+// Prep, chain and extract kernels of a prepared launch (no copies or
+// memsets: capturable in a hipGraph).
+int launch_prepared(const PreparedLaunch &L, hipStream_t stream)
+{
+    if (L.npairs == 0) return LCS_OK;
+    return lcsk_launch_chain(L.args, L.args.pairs, L.npairs, L.max_n, L.tile_m, L.blocks, L.threads, stream);
+}
+
+// Launches the band chain over `pairs` on `stream` with the context's ring,
+// descriptor buffer and ticket words.
+int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream)
+{
+    int rc = ctx->ring.reserve(ctx, pairs_ring_words(pairs) * sizeof(unsigned long long));
+    if (rc) return rc;
+    PreparedLaunch L;
+    if ((rc = prepare_pairs(ctx, pairs, tm, wps, (unsigned long long *)ctx->ring.ptr, ctx->descs, ctx->ctrl, L,
+                            stream)))
+        return rc;
+    return launch_prepared(L, stream);
+}
+
+// Strips a single pair is cut into when its bands alone would leave SIMDs
+// idle: the merge tree of the steady ant joins the strips (k = m each).
+int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
+{
+    if (m > kMaxCoreK || n < 2 * 8192) return 1;
+    const long long simds = 4LL * ctx->num_cus;
+    const long long bands = bands_of(m, n, tm);
+    int s = 1;
+    while ((long long)s * bands < simds && s < 64 && (long long)n / (2 * s) >= 8192) s *= 2;
+    return s;
+}
+
+// One pair cut into `strips` column strips: comb the strips in one chain
+// launch, turn each into its permutation, merge them pairwise with the
+// steady ant (b-split, k = m), and read the strands off the product.
+int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n, int strips, int tm,
+                   int wps, int32_t *d_h, int32_t *d_v, hipStream_t stream)
+{
+    const int32_t W = (n + strips - 1) / strips;
+    const int S = (n + W - 1) / W;
+    size_t perm_words = (size_t)S * m + n;
+    size_t ws_words = lcsb_multiply_ws_words(m + n, m + n, m);
+    const size_t words = (size_t)S * m + n + 2 * perm_words + ws_words + (size_t)(m + n) + 2 * (size_t)(S + 1);
+    int rc = ctx->strip_ws.reserve(ctx, words * 4);
+    if (rc) return rc;
+    int32_t *hs = (int32_t *)ctx->strip_ws.ptr, *vs = hs + (size_t)S * m;
+    int32_t *permA = vs + n, *permB = permA + perm_words, *mws = permB + perm_words;
+    int32_t *c2r = mws + ws_words;
+    int64_t *d_off = (int64_t *)(((uintptr_t)(c2r + m + n) + 7) & ~(uintptr_t)7);
+    std::vector<PairDesc> pairs((size_t)S);
+    std::vector<int64_t> off((size_t)S), size((size_t)S);
+    int64_t acc = 0;
+    for (int s = 0; s < S; ++s) {
+        PairDesc &P = pairs[(size_t)s];
+        P = PairDesc{};
+        P.a = d_a;
+        P.b = d_b + (size_t)s * W;
+        P.m = m;
+        P.n = std::min<int32_t>(W, n - s * W);
+        P.h_out = hs + (size_t)s * m;
+        P.v_out = vs + (size_t)s * W;
+        off[(size_t)s] = acc;
+        size[(size_t)s] = (int64_t)m + P.n;
+        acc += size[(size_t)s];
+    }
+    if ((rc = enqueue_pairs(ctx, pairs, tm, wps, stream))) return rc;
+    // enqueue_pairs staged the descriptors into ctx->descs; reuse them for the permutations
+    if ((rc = upload(ctx, d_off, off.data(), off.size() * sizeof(int64_t), stream))) return rc;
+    if ((rc = lcsk_launch_from_strands((const PairDesc *)ctx->descs.ptr, S, m + W, permA, d_off, stream))) return rc;
+    // merge tree
+    std::vector<int32_t *> cur((size_t)S);
+    std::vector<int64_t> cur_size = size;
+    for (int s = 0; s < S; ++s) cur[(size_t)s] = permA + off[(size_t)s];
+    int32_t *dst_base = permB;
+    while (cur.size() > 1) {
+        std::vector<int32_t *> nxt;
+        std::vector<int64_t> nxt_size;
+        int32_t *dst = dst_base;
+        for (size_t i = 0; i + 1 < cur.size(); i += 2) {
+            const int64_t out_size = cur_size[i] + cur_size[i + 1] - m;
+            rc = lcsb_multiply_device(cur[i], (int32_t)cur_size[i], cur[i + 1], (int32_t)cur_size[i + 1], m, 0, dst,
+                                      mws, stream);
+            if (rc) return rc;
+            nxt.push_back(dst);
+            nxt_size.push_back(out_size);
+            dst += out_size;
+        }
+        if (cur.size() % 2) {
+            // carry the odd permutation over to the next buffer
+            if (hipMemcpyAsync(dst, cur.back(), (size_t)cur_size.back() * 4, hipMemcpyDeviceToDevice, stream) !=
+                hipSuccess)
+                return LCS_EIO;
+            nxt.push_back(dst);
+            nxt_size.push_back(cur_size.back());
+        }
+        cur.swap(nxt);
+        cur_size.swap(nxt_size);
+        dst_base = (dst_base == permB) ? permA : permB;
+    }
+    return lcsk_launch_strands_from_perm(cur[0], m, n, c2r, d_h, d_v, stream);
+}
+
+// Enqueue one pair on `stream` with device pointers, following `sched`.
 int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
                   const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
                   const int32_t *d_h_in, const int32_t *d_v_in)
@@ -76,34 +268,33 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
         }
         return lcsk_launch_identity(d_h, m, d_v, n, stream);
     }
-    int tm = (sched && sched->tile_m) ? sched->tile_m : default_tile_m(ctx, m);
-    if (!valid_tile_m(tm)) return LCS_EINVAL;
-    int wps = (sched && sched->waves_per_simd) ? sched->waves_per_simd : 1;
+    const int strips_req = sched ? sched->strips : 0;
+    int tm = (sched && sched->tile_m) ? sched->tile_m : 0;
+    if (tm && !valid_tile_m(tm)) return LCS_EINVAL;
+    const int wps = (sched && sched->waves_per_simd) ? sched->waves_per_simd : 1;
     if (wps < 1 || wps > 8) return LCS_EINVAL;
-    const long long rows_per_band = 64LL * tm;
-    const long long nbands = ((long long)m + rows_per_band - 1) / rows_per_band;
-    int rc = ctx->ring.reserve(ctx, (2ull * (size_t)n + 64) * sizeof(unsigned long long));
-    if (rc) return rc;
-    ChainArgs A{};
-    A.a = d_a;
-    A.b = d_b;
-    A.m = m;
-    A.n = n;
-    A.nbands = (int32_t)nbands;
-    A.h_out = d_h;
-    A.v_out = d_v;
-    A.h_in = d_h_in;
-    A.v_in = d_v_in;
-    A.E = (unsigned long long *)ctx->ring.ptr;
-    A.ticket = ctx->ctrl;
-    A.abort_flag = ctx->ctrl + 1;
-    A.timeout_ticks = ctx->timeout_ticks;
-    A.trace = ctx->band_trace;
-    A.trace_cap = ctx->band_trace_cap;
-    const int waves_per_block = 4;
-    long long waves = std::min<long long>(nbands, 4LL * ctx->num_cus * wps);
-    int blocks = (int)((waves + waves_per_block - 1) / waves_per_block);
-    return lcsk_launch_chain(A, tm, blocks, 64 * waves_per_block, stream);
+    if (strips_req < 0) return LCS_EINVAL;
+    if (!tm) {
+        tm = default_tile_m(ctx, m);
+        if (strips_req != 1 && !d_h_in && !d_v_in && tm < 4 && default_strips(ctx, m, n, 4) > 1) tm = 4;
+    }
+    int strips = strips_req ? strips_req : ((d_h_in || d_v_in) ? 1 : default_strips(ctx, m, n, tm));
+    if (strips > 1 && (d_h_in || d_v_in)) return LCS_EINVAL;  // fixups use explicit strands: one strip
+    if (strips > 1 && m > kMaxCoreK) return LCS_ERANGE;
+    if (strips > n) strips = n;
+    if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, d_h, d_v, stream);
+    std::vector<PairDesc> pairs(1);
+    PairDesc &P = pairs[0];
+    P = PairDesc{};
+    P.a = d_a;
+    P.b = d_b;
+    P.h_in = d_h_in;
+    P.v_in = d_v_in;
+    P.h_out = d_h;
+    P.v_out = d_v;
+    P.m = m;
+    P.n = n;
+    return enqueue_pairs(ctx, pairs, tm, wps, stream);
 }
 
 int check_abort(lcs_ctx *ctx)
@@ -142,7 +333,8 @@ int lcs_ctx_create(int device, size_t hbm_budget, lcs_ctx **out)
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { delete ctx; return LCS_EIO; }
     ctx->budget = hbm_budget ? hbm_budget : free_b / 4 * 3;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return LCS_EIO; }
-    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess) {
         lcs_ctx_destroy(ctx);
         return LCS_EIO;
     }
@@ -161,6 +353,10 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     destroy_extras(ctx);
     ctx->ring.release(ctx);
+    ctx->descs.release(ctx);
+    ctx->strip_ws.release(ctx);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
     ctx->in_a.release(ctx);
     ctx->in_b.release(ctx);
     ctx->out_h.release(ctx);

@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
+#include "comb_kernels.h"
 #include "lcs_hip.h"
 
 namespace lcs {
@@ -16,7 +18,13 @@ struct DevBuf {
     void release(lcs_ctx *ctx);
 };
 
-struct Extras;  // batch / braid / server resources, created on first use
+struct Extras;  // batch / server resources, created on first use
+
+// A chain launch whose descriptors are already on the device.
+struct PreparedLaunch {
+    ChainArgs args{};
+    int blocks = 0, threads = 0, npairs = 0, max_n = 0, tile_m = 0;
+};
 
 }  // namespace lcs
 
@@ -32,8 +40,12 @@ struct lcs_ctx {
     unsigned long long timeout_ticks = 500000000ull;  // 5 s at 100 MHz
     double last_kernel_ms = 0.0;
     bool pending_timing = false;
-    lcs::DevBuf ring, in_a, in_b, out_h, out_v;
+    lcs::DevBuf ring, in_a, in_b, out_h, out_v, descs, strip_ws;
+    hipEvent_t upload_done = nullptr;    // last copy out of `pinned`
+    void *pinned = nullptr;              // pinned staging for descriptor uploads
+    size_t pinned_bytes = 0;
     lcs::Extras *extras = nullptr;
+    struct lcs_mlp *cost_model = nullptr;   // not owned
     unsigned long long *band_trace = nullptr;  // device buffer set by lcs_debug_band_trace
     int32_t band_trace_cap = 0;
 };
@@ -44,5 +56,14 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
                   const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
                   const int32_t *d_h_in = nullptr, const int32_t *d_v_in = nullptr);
 int check_abort(lcs_ctx *ctx);
+int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream);
+size_t pairs_ring_words(const std::vector<PairDesc> &pairs);
+int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
+                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream);
+int launch_prepared(const PreparedLaunch &L, hipStream_t stream);
+int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream);
+int default_batch_tile_m(int32_t m);
+bool valid_tile_m(int tm);
+int32_t bands_of(int32_t m, int32_t n, int tm);
 void destroy_extras(lcs_ctx *ctx);
 }  // namespace lcs

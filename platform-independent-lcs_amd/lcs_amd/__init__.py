@@ -73,6 +73,17 @@ def lib() -> ctypes.CDLL:
     L.lcs_permutation_hash.restype = i64
     L.lcs_braid_multiply.argtypes = [vp, _i32p, i32, _i32p, i32, i32, i32, _i32p]
     L.lcs_batch_comb.argtypes = [vp, i32, _i32p, _i64p, _i32p, _i32p, _i64p, _i32p, sched_p, i32, _i32p, _i32p]
+    L.lcs_debug_band_trace.argtypes = [vp, i32, vp]
+    f32p = ctypes.POINTER(ctypes.c_float)
+    L.lcs_mlp_create.argtypes = [vp, i32, i32, i32, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.lcs_mlp_destroy.argtypes = [vp]
+    L.lcs_mlp_param_count.argtypes = [vp, _i32p]
+    L.lcs_mlp_get_params.argtypes = [vp, f32p]
+    L.lcs_mlp_set_params.argtypes = [vp, f32p]
+    L.lcs_mlp_predict.argtypes = [vp, f32p, i32, f32p]
+    L.lcs_mlp_loss_grad.argtypes = [vp, f32p, f32p, f32p, i32, f32p, f32p]
+    L.lcs_mlp_train.argtypes = [vp, f32p, f32p, f32p, i32, i32, ctypes.c_float, f32p]
+    L.lcs_mlp_choose.argtypes = [vp, f32p, i32, _i32p]
     # workloads
     L.lcs_gen_binary_mt19937.argtypes = [i32, i32, _i32p]
     L.lcs_gen_uniform.argtypes = [ctypes.c_uint64, i32, i32, _i32p]
@@ -190,6 +201,75 @@ class Context:
         _check("lcs_batch_comb", rc)
         return ([h[a_off[p]: a_off[p] + m[p]].copy() for p in range(npairs)],
                 [v[b_off[p]: b_off[p] + n[p]].copy() for p in range(npairs)])
+
+
+def _f(x: np.ndarray):
+    return x.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+
+
+class Mlp:
+    """The schedule cost model (lcs_mlp_*): fp32 MLP trained on the GPU."""
+
+    def __init__(self, ctx: Context, n_in: int = 4, hidden: int = 64, n_out: int = 6, seed: int = 1):
+        self.ctx = ctx
+        self.n_in, self.hidden, self.n_out = n_in, hidden, n_out
+        self._h = ctypes.c_void_p()
+        _check("lcs_mlp_create", lib().lcs_mlp_create(ctx.handle, n_in, hidden, n_out, seed, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().lcs_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def param_count(self) -> int:
+        c = ctypes.c_int32()
+        _check("lcs_mlp_param_count", lib().lcs_mlp_param_count(self._h, ctypes.byref(c)))
+        return c.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count(), np.float32)
+        _check("lcs_mlp_get_params", lib().lcs_mlp_get_params(self._h, _f(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        assert p.size == self.param_count()
+        _check("lcs_mlp_set_params", lib().lcs_mlp_set_params(self._h, _f(p)))
+
+    def predict(self, X) -> np.ndarray:
+        X = _f32(X).reshape(-1, self.n_in)
+        Y = np.empty((max(X.shape[0], 1), self.n_out), np.float32)
+        _check("lcs_mlp_predict", lib().lcs_mlp_predict(self._h, _f(X), X.shape[0], _f(Y)))
+        return Y[: X.shape[0]]
+
+    def loss_grad(self, X, T, M):
+        X, T, M = _f32(X), _f32(T), _f32(M)
+        loss = np.zeros(1, np.float32)
+        g = np.empty(self.param_count(), np.float32)
+        _check("lcs_mlp_loss_grad", lib().lcs_mlp_loss_grad(self._h, _f(X), _f(T), _f(M), X.shape[0], _f(loss), _f(g)))
+        return float(loss[0]), g
+
+    def train(self, X, T, M, steps: int, lr: float = 3e-3) -> float:
+        X, T, M = _f32(X), _f32(T), _f32(M)
+        loss = np.zeros(1, np.float32)
+        _check("lcs_mlp_train", lib().lcs_mlp_train(self._h, _f(X), _f(T), _f(M), X.shape[0], steps, lr, _f(loss)))
+        return float(loss[0])
+
+    def choose(self, X) -> np.ndarray:
+        X = _f32(X).reshape(-1, self.n_in)
+        c = np.empty(max(X.shape[0], 1), np.int32)
+        _check("lcs_mlp_choose", lib().lcs_mlp_choose(self._h, _f(X), X.shape[0], _p(c)))
+        return c[: X.shape[0]]
 
 
 def from_strands(h, v):

@@ -101,3 +101,21 @@ def test_repeatable(ctx):
     r1 = ctx.comb(w.a, w.b)
     r2 = ctx.comb(w.a, w.b, lcs_amd.schedule(tile_m=4, waves_per_simd=2))
     assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+
+
+@pytest.mark.parametrize("strips", [2, 3, 8])
+def test_column_strips_merged_by_steady_ant(ctx, strips):
+    rng = np.random.default_rng(strips)
+    a = rng.integers(0, 4, 3000).astype(np.int32)
+    b = rng.integers(0, 4, 40000).astype(np.int32)
+    _check(ctx, a, b, lcs_amd.schedule(tile_m=4, strips=strips))
+
+
+def test_thin_pair_default_schedule(ctx):
+    # a thin, wide pair like the Zipf workload (reduced): the default
+    # schedule cuts it into strips and merges them on the GPU
+    a = lcs_amd.gen_zipf(5, 1 << 12, 26, 1.1)
+    b = lcs_amd.gen_planted(6, a, 1 << 18, 26, 1.1, 16, 0.05)
+    h, v, _ = ctx.comb(a, b)
+    ho, vo = oracle.comb(a, b, order="mt", threads=8)
+    assert np.array_equal(h, ho) and np.array_equal(v, vo)
