@@ -134,6 +134,46 @@ int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs,
                    const lcs_schedule *scheds, int32_t use_graph,
                    int32_t *h_out, int32_t *v_out);
 
+/* ---- autotuner ------------------------------------------------------------
+ * No counterpart in the reference (its variants were swept offline by
+ * notebooks, tests/test_prepare.ipynb).  Times each candidate schedule on
+ * synthetic pairs of the shape (A uniform over `alphabet`, B = A mutated at
+ * `mutation_rate`), best of `reps` runs after a warm-up: mode 0 times one
+ * pair through lcs_comb_device (cost = kernel ms); mode 1 times a batch of
+ * identical-shape pairs through lcs_batch_comb (cost = kernel ms per pair).
+ * cost_ms has ncand entries. */
+int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t alphabet, float mutation_rate, uint64_t seed,
+                         const lcs_schedule *candidates, int32_t ncand, int32_t mode, int32_t reps, float *cost_ms);
+
+/* ---- request server ------------------------------------------------------
+ * No counterpart in the reference.  Continuous batching of pair comparisons:
+ * a worker thread admits queued requests, oldest first, into a device arena
+ * of hbm_budget bytes (0 = half the free memory) as memory and one of
+ * max_inflight (1..8) micro-batch slots free up; micro-batches run on their
+ * own streams and overlap.  Inputs are copied at submit; results stay until
+ * lcs_server_release.  Each result equals lcs_comb's for the same pair.
+ * features (nullable, 4 floats: the cost model's inputs) let an attached
+ * model choose the schedule when sched is NULL or has tile_m == 0. */
+typedef struct lcs_server lcs_server;
+int lcs_server_create(lcs_ctx *ctx, size_t hbm_budget, int32_t max_inflight, lcs_server **out);
+int lcs_server_destroy(lcs_server *server);
+int lcs_server_submit(lcs_server *server, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+                      const lcs_schedule *sched, const float *features, int64_t *request_id);
+/* *done = 1 and the strands copied out when finished (0 = still queued or
+ * running); returns the request's status once done */
+int lcs_server_poll(lcs_server *server, int64_t request_id, int32_t *h_out, int32_t *v_out, int32_t *done);
+/* blocks up to timeout_ms (< 0: forever); LCS_EAGAIN on timeout */
+int lcs_server_wait(lcs_server *server, int64_t request_id, int32_t *h_out, int32_t *v_out, int32_t timeout_ms);
+int lcs_server_cancel(lcs_server *server, int64_t request_id);
+/* forget a finished request (its result and times) */
+int lcs_server_release(lcs_server *server, int64_t request_id);
+/* microseconds from submit to admission and from submit to result */
+int lcs_server_request_times(lcs_server *server, int64_t request_id, double *queued_us, double *total_us);
+/* queue length, completed requests, launched micro-batches, most slots in
+ * flight at once, free arena bytes; returns the worker's last error */
+int lcs_server_stats(lcs_server *server, int64_t *pending, int64_t *completed, int64_t *batches,
+                     int32_t *max_inflight, size_t *free_bytes);
+
 /* ---- learned schedule cost model ---------------------------------------
  * No counterpart in the reference (its variant choice is a compile-time
  * name, src/algorithm_registry.cpp:127-142).  An fp32 MLP
@@ -158,6 +198,10 @@ int lcs_mlp_train(lcs_mlp *mlp, const float *X, const float *T, const float *M, 
                   float *final_loss);
 /* index of the smallest predicted cost per row, computed on the GPU */
 int lcs_mlp_choose(lcs_mlp *mlp, const float *X, int32_t n, int32_t *choice);
+/* The server chooses schedules of requests submitted with features (and no
+ * tile_m) among candidates[0..ncand) with the model (NULL detaches it).
+ * Declared after lcs_mlp. */
+int lcs_server_set_cost_model(lcs_server *server, lcs_mlp *model, const lcs_schedule *candidates, int32_t ncand);
 
 #ifdef __cplusplus
 }

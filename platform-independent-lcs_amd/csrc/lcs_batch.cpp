@@ -88,7 +88,9 @@ int run_launches(lcs_ctx *ctx, Extras *X, const std::vector<int> &classes,
                  const std::vector<PreparedLaunch> &launches, bool use_graph)
 {
     hipStream_t s = ctx->stream;
-    if (hipEventRecord(ctx->ev0, s) != hipSuccess) return LCS_EIO;
+    // timing brackets the kernels only: in the graph path ev0 is recorded
+    // after instantiation, right before the graph launch
+    if (!use_graph && hipEventRecord(ctx->ev0, s) != hipSuccess) return LCS_EIO;
     hipGraph_t graph = nullptr;
     if (use_graph && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return LCS_EIO;
     int rc = LCS_OK;
@@ -107,14 +109,15 @@ int run_launches(lcs_ctx *ctx, Extras *X, const std::vector<int> &classes,
             hipGraphExec_t exec = nullptr;
             if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = LCS_EIO;
             else {
-                if (hipGraphLaunch(exec, s) != hipSuccess) rc = LCS_EIO;
+                if (hipEventRecord(ctx->ev0, s) != hipSuccess || hipGraphLaunch(exec, s) != hipSuccess) rc = LCS_EIO;
+                if (!rc && hipEventRecord(ctx->ev1, s) != hipSuccess) rc = LCS_EIO;
                 if (hipStreamSynchronize(s) != hipSuccess) rc = LCS_EIO;
                 (void)hipGraphExecDestroy(exec);
             }
         }
         if (graph) (void)hipGraphDestroy(graph);
     }
-    if (!rc && hipEventRecord(ctx->ev1, s) != hipSuccess) rc = LCS_EIO;
+    if (!use_graph && !rc && hipEventRecord(ctx->ev1, s) != hipSuccess) rc = LCS_EIO;
     return rc;
 }
 

@@ -117,8 +117,10 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     for (auto &P : pairs) {
         P.tile_m = tm;
         P.nbands = bands_of(P.m, P.n, tm);
-        P.E = E;
-        E += lcsk_ring_words(P.n);
+        if (ring) {  // else every pair already carries its own ring
+            P.E = E;
+            E += lcsk_ring_words(P.n);
+        }
         max_bands = std::max(max_bands, P.nbands);
         max_n = std::max(max_n, std::max(P.n, P.m));
     }
@@ -150,6 +152,12 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     L.max_n = max_n;
     L.tile_m = tm;
     return LCS_OK;
+}
+
+int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, DevBuf &descs,
+                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream)
+{
+    return prepare_pairs(ctx, pairs, tm, wps, nullptr, descs, ctrl, L, stream);
 }
 
 // Prep, chain and extract kernels of a prepared launch (no copies or

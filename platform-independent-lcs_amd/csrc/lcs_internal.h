@@ -61,6 +61,8 @@ size_t pairs_ring_words(const std::vector<PairDesc> &pairs);
 int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
                   DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream);
 int launch_prepared(const PreparedLaunch &L, hipStream_t stream);
+int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, DevBuf &descs,
+                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream);
 int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream);
 int default_batch_tile_m(int32_t m);
 bool valid_tile_m(int tm);

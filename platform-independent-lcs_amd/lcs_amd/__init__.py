@@ -84,6 +84,18 @@ def lib() -> ctypes.CDLL:
     L.lcs_mlp_loss_grad.argtypes = [vp, f32p, f32p, f32p, i32, f32p, f32p]
     L.lcs_mlp_train.argtypes = [vp, f32p, f32p, f32p, i32, i32, ctypes.c_float, f32p]
     L.lcs_mlp_choose.argtypes = [vp, f32p, i32, _i32p]
+    L.lcs_autotune_measure.argtypes = [vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, sched_p, i32, i32, i32,
+                                       f32p]
+    L.lcs_server_create.argtypes = [vp, sz, i32, ctypes.POINTER(vp)]
+    L.lcs_server_destroy.argtypes = [vp]
+    L.lcs_server_submit.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, f32p, _i64p]
+    L.lcs_server_poll.argtypes = [vp, i64, _i32p, _i32p, _i32p]
+    L.lcs_server_wait.argtypes = [vp, i64, _i32p, _i32p, i32]
+    L.lcs_server_cancel.argtypes = [vp, i64]
+    L.lcs_server_release.argtypes = [vp, i64]
+    L.lcs_server_request_times.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.lcs_server_stats.argtypes = [vp, _i64p, _i64p, _i64p, _i32p, ctypes.POINTER(sz)]
+    L.lcs_server_set_cost_model.argtypes = [vp, vp, sched_p, i32]
     # workloads
     L.lcs_gen_binary_mt19937.argtypes = [i32, i32, _i32p]
     L.lcs_gen_uniform.argtypes = [ctypes.c_uint64, i32, i32, _i32p]
@@ -178,6 +190,15 @@ class Context:
                                                               int(bool(row_glue)), _p(out)))
         return out[: p.size + q.size - k].copy()
 
+    def autotune(self, m: int, n: int, alphabet: int, rate: float, candidates, mode: int = 1, reps: int = 2,
+                 seed: int = 1) -> np.ndarray:
+        """Measured cost (ms per pair) of each candidate schedule for the shape."""
+        arr = (Schedule * len(candidates))(*candidates)
+        out = np.empty(len(candidates), np.float32)
+        _check("lcs_autotune_measure", lib().lcs_autotune_measure(
+            self._h, m, n, alphabet, rate, seed, arr, len(candidates), mode, reps, _f(out)))
+        return out
+
     def batch_comb(self, a_list, b_list, scheds=None, use_graph: bool = True):
         """Comb every pair (a_list[p], b_list[p]) in one batched launch."""
         npairs = len(a_list)
@@ -270,6 +291,79 @@ class Mlp:
         c = np.empty(max(X.shape[0], 1), np.int32)
         _check("lcs_mlp_choose", lib().lcs_mlp_choose(self._h, _f(X), X.shape[0], _p(c)))
         return c[: X.shape[0]]
+
+
+class Server:
+    """Continuous-batching request server (lcs_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_inflight: int = 4):
+        self._h = ctypes.c_void_p()
+        self.sizes = {}
+        _check("lcs_server_create", lib().lcs_server_create(ctx.handle, hbm_budget, max_inflight,
+                                                            ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().lcs_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_cost_model(self, mlp: "Mlp | None", candidates):
+        arr = (Schedule * len(candidates))(*candidates) if candidates else None
+        _check("lcs_server_set_cost_model",
+               lib().lcs_server_set_cost_model(self._h, mlp._h if mlp else None, arr, len(candidates)))
+
+    def submit(self, a, b, sched: Schedule | None = None, features=None) -> int:
+        a, b = _i32(a), _i32(b)
+        rid = ctypes.c_int64()
+        f = _f32(features) if features is not None else None
+        _check("lcs_server_submit", lib().lcs_server_submit(
+            self._h, _p(a), a.size, _p(b), b.size, ctypes.byref(sched) if sched is not None else None,
+            _f(f) if f is not None else None, ctypes.byref(rid)))
+        self.sizes[rid.value] = (a.size, b.size)
+        return rid.value
+
+    def poll(self, rid: int):
+        m, n = self.sizes[rid]
+        h = np.empty(max(m, 1), np.int32)
+        v = np.empty(max(n, 1), np.int32)
+        done = ctypes.c_int32()
+        _check("lcs_server_poll", lib().lcs_server_poll(self._h, rid, _p(h), _p(v), ctypes.byref(done)))
+        return (h[:m].copy(), v[:n].copy()) if done.value else None
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        m, n = self.sizes[rid]
+        h = np.empty(max(m, 1), np.int32)
+        v = np.empty(max(n, 1), np.int32)
+        _check("lcs_server_wait", lib().lcs_server_wait(self._h, rid, _p(h), _p(v), timeout_ms))
+        return h[:m].copy(), v[:n].copy()
+
+    def times(self, rid: int):
+        q, t = ctypes.c_double(), ctypes.c_double()
+        _check("lcs_server_request_times", lib().lcs_server_request_times(self._h, rid, ctypes.byref(q),
+                                                                          ctypes.byref(t)))
+        return q.value, t.value
+
+    def release(self, rid: int):
+        _check("lcs_server_release", lib().lcs_server_release(self._h, rid))
+        self.sizes.pop(rid, None)
+
+    def cancel(self, rid: int):
+        _check("lcs_server_cancel", lib().lcs_server_cancel(self._h, rid))
+
+    def stats(self):
+        p, c, bt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        mi = ctypes.c_int32()
+        fb = ctypes.c_size_t()
+        rc = lib().lcs_server_stats(self._h, ctypes.byref(p), ctypes.byref(c), ctypes.byref(bt), ctypes.byref(mi),
+                                    ctypes.byref(fb))
+        return {"pending": p.value, "completed": c.value, "batches": bt.value, "max_inflight": mi.value,
+                "free_bytes": fb.value, "status": rc}
 
 
 def from_strands(h, v):
