@@ -111,6 +111,18 @@ int lcs_permutation_from_strands(const int32_t *h, int32_t m, const int32_t *v, 
  * reference's two's-complement wrap made well defined.  Host. */
 int64_t lcs_permutation_hash(const int32_t *col_to_row, int32_t size);
 
+/* Semi-local scores read from a permutation (row_to_col, host pointers) on
+ * the GPU.  The reference reads no scores from its permutations (its only
+ * score routine, Lcs_Prefix_Reference in src/lcs_reference.hpp:71-132, is
+ * an unregistered DP); these follow the seaweed read-out:
+ *   lcs_permutation_dominance:  out[q] = #{ r >= qr[q] : row_to_col[r] < qc[q] }
+ *   lcs_semilocal_string_substring: out[q] = LCS(a, b[qi[q]:qj[q])) for
+ *     0 <= qi <= qj <= n, from P(a, b) of size m+n. */
+int lcs_permutation_dominance(lcs_ctx *ctx, const int32_t *row_to_col, int32_t size, const int32_t *qr,
+                              const int32_t *qc, int32_t nq, int32_t *out);
+int lcs_semilocal_string_substring(lcs_ctx *ctx, const int32_t *row_to_col, int32_t m, int32_t n, const int32_t *qi,
+                                   const int32_t *qj, int32_t nq, int32_t *out);
+
 /* staggered_multiply<RowGlue>(p, q, k) (src/braid_multiplication.hpp:273-406)
  * on the GPU: the sticky-braid product of p (size p_size) and q (size
  * q_size) glued on k strands; row_glue = 0 glues p's last k columns to q's

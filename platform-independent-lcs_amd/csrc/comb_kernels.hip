@@ -64,11 +64,21 @@ __device__ __forceinline__ uint32_t shl1_keep(uint32_t old_, uint32_t x)
     return (uint32_t)__builtin_amdgcn_update_dpp((int)old_, (int)x, kDppWaveShl1, 0xf, 0xf, false);
 }
 
-__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p)
+// Pointers read from a pair descriptor are generic; the band loop casts them
+// to the global address space so its loads and stores are global_* rather
+// than flat_* instructions.  A flat access may hit LDS, whose results return
+// out of order, and that forces a full vmcnt(0) wait (on the in-flight ring
+// store too) where a global access needs only vmcnt(1).
+#define LCSK_GLOBAL __attribute__((address_space(1)))
+typedef LCSK_GLOBAL unsigned long long gword;
+typedef LCSK_GLOBAL const unsigned long long gcword;
+typedef LCSK_GLOBAL const int32_t gsym;
+
+__device__ __forceinline__ unsigned long long ld_relaxed(gcword *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_relaxed(unsigned long long *p, unsigned long long x)
+__device__ __forceinline__ void st_relaxed(gword *p, unsigned long long x)
 {
     __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -165,31 +175,40 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     }
     const uint32_t tag_in = (uint32_t)g;
     const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
-    const unsigned long long *Ein = P.E + (size_t)(g & 1) * (size_t)n;
-    unsigned long long *Eout = P.E + (size_t)((g + 1) & 1) * (size_t)n;
+    gcword *Ein = (gcword *)(P.E + (size_t)(g & 1) * (size_t)n);
+    gword *Eout = (gword *)(P.E + (size_t)((g + 1) & 1) * (size_t)n);
+    gsym *bsym = (gsym *)P.b;
     const int nchunks = (n + kWave + C - 1) / C;
 
     // prefetch chunk 0.  Every lane loads from a clamped in-bounds address
     // (no exec masking, no select), so the waits before each chunk cover
     // exactly these loads.
     const int nm1 = n - 1;
-    int32_t pB = P.b[min(lane, nm1)];
-    unsigned long long pE = ld_relaxed(Ein + min(lane, nm1));
+    // Only lanes < C use their words (the conveyors start at lane 0 and
+    // shift C times); lanes >= C load the same addresses, so each chunk
+    // fetches C words, not 64.
+    int32_t pB = bsym[min(lane & (C - 1), nm1)];
+    unsigned long long pE = ld_relaxed(Ein + min(lane & (C - 1), nm1));
     // Publishes the C bottom-edge strands caught in O during the chunk that
     // started at step t: O lanes [64-C, 64) hold columns t-64 .. t-64+C-1.
-    // Every lane stores every time (lanes without a column store to a trash
-    // word past the ring), so no path skips the store and the waits for the
-    // prefetch loads can leave it in flight.
-    unsigned long long *trash = P.E + 2 * (size_t)n + lane;
+    // Every lane stores every time, so no path skips the store and the waits
+    // for the prefetch loads can leave it in flight: lane L stores the word of
+    // lane 64-C + L%C (fetched by one bpermute), so the C/64 copies of each
+    // word carry the same value to the same address.  Columns outside [0, n)
+    // (the first and last chunks of a band) go to trash words past the ring.
+    static_assert(C > 0 && (C & (C - 1)) == 0 && C <= kWave, "C must be a power of two <= 64");
+    gword *trash = (gword *)(P.E + 2 * (size_t)n);
     auto publish = [&](uint32_t Ox, int t) {
-        const int col = t + C - 2 * kWave + lane;
-        const bool mine = lane >= kWave - C && col >= 0 && col < n;
-        st_relaxed(mine ? Eout + col : trash, tag_out | (unsigned long long)Ox);
+        const int src = kWave - C + (lane & (C - 1));
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)Ox);
+        const int col = t + C - 2 * kWave + src;
+        const bool mine = col >= 0 && col < n;
+        st_relaxed(mine ? Eout + col : trash + src, tag_out | (unsigned long long)x);
     };
     // Turns the prefetched ring words of the chunk starting at t into lane 0's
     // conveyor I (waiting, bounded in time, until they carry this band's tag).
     auto take_inputs = [&](int t, uint32_t &I) -> bool {
-        const int c = t + lane;
+        const int c = t + (lane & (C - 1));   // lanes >= C repeat lanes < C
         const bool need = lane < C && c < n;
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
@@ -216,8 +235,8 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
         const int t0 = k * C;
         {   // prefetch chunk k+1, then publish chunk k-1: the wait for these
             // loads at the end of the body then leaves the store in flight
-            const int c = min(t0 + C + lane, nm1);
-            pB = P.b[c];
+            const int c = min(t0 + C + (lane & (C - 1)), nm1);
+            pB = bsym[c];
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C);

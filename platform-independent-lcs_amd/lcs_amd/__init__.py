@@ -86,6 +86,8 @@ def lib() -> ctypes.CDLL:
     L.lcs_mlp_choose.argtypes = [vp, f32p, i32, _i32p]
     L.lcs_autotune_measure.argtypes = [vp, i32, i32, i32, ctypes.c_float, ctypes.c_uint64, sched_p, i32, i32, i32,
                                        f32p]
+    L.lcs_permutation_dominance.argtypes = [vp, _i32p, i32, _i32p, _i32p, i32, _i32p]
+    L.lcs_semilocal_string_substring.argtypes = [vp, _i32p, i32, i32, _i32p, _i32p, i32, _i32p]
     L.lcs_server_create.argtypes = [vp, sz, i32, ctypes.POINTER(vp)]
     L.lcs_server_destroy.argtypes = [vp]
     L.lcs_server_submit.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, f32p, _i64p]
@@ -198,6 +200,21 @@ class Context:
         _check("lcs_autotune_measure", lib().lcs_autotune_measure(
             self._h, m, n, alphabet, rate, seed, arr, len(candidates), mode, reps, _f(out)))
         return out
+
+    def string_substring(self, row_to_col, m: int, n: int, qi, qj) -> np.ndarray:
+        """LCS(a, b[qi:qj)) per query, read from P(a, b) on the GPU."""
+        p, qi, qj = _i32(row_to_col), _i32(qi), _i32(qj)
+        out = np.empty(max(qi.size, 1), np.int32)
+        _check("lcs_semilocal_string_substring", lib().lcs_semilocal_string_substring(
+            self._h, _p(p), m, n, _p(qi), _p(qj), qi.size, _p(out)))
+        return out[: qi.size].copy()
+
+    def dominance(self, row_to_col, qr, qc) -> np.ndarray:
+        p, qr, qc = _i32(row_to_col), _i32(qr), _i32(qc)
+        out = np.empty(max(qr.size, 1), np.int32)
+        _check("lcs_permutation_dominance", lib().lcs_permutation_dominance(
+            self._h, _p(p), p.size, _p(qr), _p(qc), qr.size, _p(out)))
+        return out[: qr.size].copy()
 
     def batch_comb(self, a_list, b_list, scheds=None, use_graph: bool = True):
         """Comb every pair (a_list[p], b_list[p]) in one batched launch."""
