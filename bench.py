@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--log2n", type=int, default=20, help="m = n = 2^log2n")
     ap.add_argument("--tile-m", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 register chain, 2 LDS band groups")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="cross-check two schedules bit-for-bit")
@@ -91,7 +92,7 @@ def main():
     m, n = a.size, b.size
     ctx = lcs_amd.Context(0)
     cus, _ = ctx.info()
-    sched = lcs_amd.schedule(tile_m=args.tile_m, waves_per_simd=args.waves_per_simd)
+    sched = lcs_amd.schedule(tile_m=args.tile_m, waves_per_simd=args.waves_per_simd, variant=args.variant)
     da, db = H.DeviceArray.from_numpy(a), H.DeviceArray.from_numpy(b)
     dh, dv = H.DeviceArray(m), H.DeviceArray(n)
     L = lcs_amd.lib()
@@ -143,7 +144,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded mt19937_64: uniform alphabet-4 A, B = A + 10% subs + 2% indels)",
         "config": {"workload": f"single pair m=n=2^{args.log2n}, tiled band-chain comb", "m": m, "n": n,
-                   "alphabet": 4, "tile_m": args.tile_m or None, "llcs": llcs},
+                   "alphabet": 4, "tile_m": args.tile_m or None, "variant": args.variant or None, "llcs": llcs},
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 3), "peak": round(peak_ops / 1e12, 3),
                      "unit": "TOP/s (int32 lane-ops, 4 per cell)", "frac": round(achieved_ops / peak_ops, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 3),

@@ -22,12 +22,14 @@ struct PairDesc {
 inline size_t lcsk_ring_words(int32_t n) { return 2 * (size_t)n + 64; }
 
 // One persistent chain launch: waves take tickets in order; ticket t runs
-// band tasks[t].y of pair tasks[t].x.  Tasks must list every band after the
+// band tasks[t].y of pair tasks[t].x (group_w == 1), or bands
+// group_w*tasks[t].y .. +group_w-1 in one workgroup (group_w > 1).  Tasks must list every band after the
 // band above it (same pair), which makes every wait one on a resident wave.
 struct ChainArgs {
     const PairDesc *pairs;
     const int2 *tasks;
-    int32_t ntasks, pad0;
+    int32_t ntasks;
+    int32_t group_w;             // 1: one wave per task (register chain); W > 1: LDS groups of W bands
     unsigned *ticket;            // zeroed by the prep kernel
     unsigned *abort_flag;        // zeroed by the prep kernel
     unsigned long long timeout_ticks;  // 100 MHz realtime ticks per wait

@@ -73,6 +73,11 @@ __device__ __forceinline__ uint32_t shl1_keep(uint32_t old_, uint32_t x)
 typedef LCSK_GLOBAL unsigned long long gword;
 typedef LCSK_GLOBAL const unsigned long long gcword;
 typedef LCSK_GLOBAL const int32_t gsym;
+typedef LCSK_GLOBAL int32_t gout;
+// LDS words polled through a pointer: the explicit address space keeps the
+// reads ds_read (a volatile generic pointer would be a flat access, and a
+// flat access waits for every outstanding global store).
+typedef __attribute__((address_space(3))) volatile unsigned lds_flag;
 
 __device__ __forceinline__ unsigned long long ld_relaxed(gcword *p)
 {
@@ -278,6 +283,277 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
     }
 }
 
+// ---- LDS group variant -----------------------------------------------------
+//
+// W consecutive bands of one pair share a workgroup, one wave per band (one
+// per SIMD of a CU).  Inside the group the bottom-edge strands go from wave w
+// to wave w+1 through an R-slot LDS ring guarded by two progress words (the
+// producer's last published chunk, the consumer's last consumed chunk), and
+// the b symbols are loaded once per group by wave 0 (coalesced, 64 B per
+// chunk) into an LDS ring that every wave reads.  Only the group's first wave
+// reads the global ring and only its last wave writes it.  Waves are not
+// lock-stepped: each waits only for the data it needs and for a free slot,
+// so the ring gives the same slack the global ring gives between bands,
+// while a hand-off costs an LDS round trip instead of a global one.  A wave
+// finishes chunk j-kShift's columns (kShift = 64/C: the lane skew) during its
+// chunk j.  Every wait is bounded in time; an expired wait raises the global
+// abort flag and an LDS stop word that every waiting wave checks.
+constexpr int kXferSlots = 8;   // LDS ring between consecutive waves (chunks)
+constexpr int kBRing = 64;      // b chunks staged in LDS
+
+struct GroupLds {
+    volatile int prog[4];   // prog[w]: last column chunk wave w published (-1: none)
+    volatile int cons[4];   // cons[w]: last chunk wave w took from wave w-1
+    volatile unsigned stop; // set by a wave whose wait expired
+};
+typedef __attribute__((address_space(3))) GroupLds lds_group;
+
+// Spins until *word >= target (LDS), the stop word is set, or the time runs
+// out (then raises both abort words).  Returns false when the wave must stop.
+__device__ __forceinline__ bool lds_wait_ge(__attribute__((address_space(3))) volatile int *word, int target,
+                                            lds_group *G, const ChainArgs &A)
+{
+    if (*word >= target) return true;
+    const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+    while (*word < target) {
+        if (G->stop) return false;
+        if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+            G->stop = 1u;
+            raise_abort(A.abort_flag);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// ROLE bit 1: the group's first wave (reads the global ring, stages b);
+// bit 2: its last wave (writes the global ring).  The role is a template
+// argument so each wave's loop is straight-line code: with the role as a
+// runtime branch the waits of the merged paths cover every outstanding load.
+constexpr int kFirst = 1, kLast = 2;
+
+template <int TM, int C, int W, bool ROWMASK, int ROLE>
+__device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, int lane,
+                           uint32_t (*xfer_)[kXferSlots][C], int32_t (*bring_)[C], lds_group *G)
+{
+    constexpr int kShift = kWave / C;
+    static_assert((W - 1) * (kShift + kXferSlots + 1) + 2 <= kBRing, "b ring too short for the group's spread");
+    constexpr bool first = (ROLE & kFirst) != 0, last = (ROLE & kLast) != 0;
+    // Every LDS access of the hand-off is volatile: the compiler keeps them
+    // in program order and the LDS executes one wave's accesses in order, so
+    // a progress word read before a data word vouches for that data.
+    typedef __attribute__((address_space(3))) volatile uint32_t lds_u32;
+    typedef __attribute__((address_space(3))) volatile int32_t lds_i32;
+    lds_u32 *xfer_in = (lds_u32 *)(w > 0 ? &xfer_[w > 0 ? w - 1 : 0][0][0] : &xfer_[0][0][0]);
+    lds_u32 *xfer_out = (lds_u32 *)&xfer_[last ? 0 : w][0][0];
+    lds_i32 *bring = (lds_i32 *)&bring_[0][0];
+    const int m = P.m, n = P.n;
+    const int r0 = g * (kWave * TM) + lane * TM;
+    int32_t a[TM];
+    uint32_t h[TM];
+    bool valid[TM];
+    gsym *asym = (gsym *)P.a, *hin = (gsym *)P.h_in;
+#pragma unroll
+    for (int k = 0; k < TM; ++k) {
+        const int r = r0 + k;
+        valid[k] = ROWMASK ? (r < m) : true;
+        a[k] = valid[k] ? asym[r] : 0;
+        h[k] = valid[k] ? (hin ? (uint32_t)hin[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
+    }
+    // Let the row loads land here: left outstanding into the loop, the wait
+    // for them would sit before the cells of every chunk and also cover the
+    // last wave's in-flight ring store.
+    __builtin_amdgcn_s_waitcnt(0);
+    const uint32_t tag_in = (uint32_t)g;
+    const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
+    gcword *Ein = (gcword *)(P.E + (size_t)(g & 1) * (size_t)n);
+    gword *Eout = (gword *)(P.E + (size_t)((g + 1) & 1) * (size_t)n);
+    gword *trash = (gword *)(P.E + 2 * (size_t)n);
+    gsym *bsym = (gsym *)P.b;
+    const int nchunks = (n + kWave + C - 1) / C;
+    const int nin = (n + C - 1) / C;   // chunks whose columns carry inputs
+    const int nm1 = n - 1;
+    const int ql = lane & (C - 1);
+    // as in band_pass: every lane stores, C/64 copies of each word
+    auto publish = [&](uint32_t Ox, int t) {
+        const int src = kWave - C + ql;
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)Ox);
+        const int col = t + C - 2 * kWave + src;
+        st_relaxed(col >= 0 && col < n ? Eout + col : trash + src, tag_out | (unsigned long long)x);
+    };
+    unsigned long long pE = 0;
+    auto take_inputs = [&](int t, uint32_t &I) -> bool {
+        const int c = t + ql;
+        const bool need = lane < C && c < n;
+        unsigned long long e = pE;
+        bool ok = !need || (uint32_t)(e >> 32) == tag_in;
+        if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+            const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
+            do {
+                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
+                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                    G->stop = 1u;
+                    raise_abort(A.abort_flag);
+                    return false;
+                }
+                if (!ok) e = ld_relaxed(Ein + min(c, nm1));
+                ok = !need || (uint32_t)(e >> 32) == tag_in;
+            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
+        }
+        I = (uint32_t)e;
+        return true;
+    };
+    // chunk j's inputs from wave w-1 (wave >= 1): read now, checked later
+    int32_t pn = 0;
+    uint32_t In = 0;
+    int32_t IBn = 0;
+    auto lds_prefetch = [&](int j) {
+        pn = G->prog[w - 1];
+        In = xfer_in[(j & (kXferSlots - 1)) * C + ql];
+        IBn = bring[(j & (kBRing - 1)) * C + ql];
+    };
+    auto lds_take = [&](int j, uint32_t &I, int32_t &IB) -> bool {
+        if (__builtin_amdgcn_readfirstlane(pn) < j) {
+            if (!lds_wait_ge(&G->prog[w - 1], j, G, A)) return false;
+            In = xfer_in[(j & (kXferSlots - 1)) * C + ql];
+            IBn = bring[(j & (kBRing - 1)) * C + ql];
+        }
+        I = In;
+        IB = IBn;
+        if (lane == 0) G->cons[w] = j;
+        return true;
+    };
+    int32_t cn = 0;   // consumer word of wave w+1, read a chunk ahead
+
+    uint32_t v = 0, O = 0, Osave = 0, I = 0;
+    int32_t bq = 0, IB = 0, pB = 0;
+    if constexpr (first) {
+        pB = bsym[min(ql, nm1)];
+        pE = ld_relaxed(Ein + min(ql, nm1));
+        if (!take_inputs(0, I)) return false;
+        IB = pB;
+        if (lane < C) bring[lane] = pB;
+    } else {
+        lds_prefetch(0);
+        if (!lds_take(0, I, IB)) return false;
+    }
+    for (int j = 0; j < nchunks; ++j) {
+        const int t0 = j * C;
+        // Order: global loads, then the ring store (so the wait for the
+        // loads at the end of the chunk leaves the store in flight), then the
+        // LDS prefetch (so the store's bpermute wait covers none of it).
+        if constexpr (first) {
+            if (j + 1 < nchunks) {
+                const int c = min(t0 + C + ql, nm1);
+                pB = bsym[c];
+                pE = ld_relaxed(Ein + c);
+            }
+        }
+        if constexpr (last) publish(Osave, t0 - C);
+        if constexpr (!first) {
+            if (j + 1 < nin) lds_prefetch(j + 1);
+        }
+        if constexpr (!last) cn = G->cons[w + 1];
+        if (t0 >= kWave - 1 && t0 + C <= n)
+            band_steps<TM, C, ROWMASK, false>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
+        else
+            band_steps<TM, C, ROWMASK, true>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
+        Osave = O;
+        if constexpr (!last) {
+            if (j >= kShift && j - kShift < nin) {
+                const int jo = j - kShift;   // the column chunk O now completes
+                if (__builtin_amdgcn_readfirstlane(cn) < jo - kXferSlots &&
+                    !lds_wait_ge(&G->cons[w + 1], jo - kXferSlots, G, A))
+                    return false;
+                if (lane >= kWave - C) xfer_out[(jo & (kXferSlots - 1)) * C + lane - (kWave - C)] = O;
+                if (lane == 0) G->prog[w] = jo;
+            }
+        }
+        if constexpr (first) {
+            if (j + 1 < nchunks) {
+                IB = pB;
+                if (!take_inputs(t0 + C, I)) return false;
+                if (lane < C) bring[((j + 1) & (kBRing - 1)) * C + lane] = pB;
+            }
+        } else {
+            if (j + 1 < nin) {
+                if (!lds_take(j + 1, I, IB)) return false;
+            } else {
+                I = 0;   // columns past n: no input
+                IB = 0;
+            }
+        }
+    }
+    if constexpr (last) publish(Osave, (nchunks - 1) * C);
+    gout *hout = (gout *)P.h_out;
+#pragma unroll
+    for (int k = 0; k < TM; ++k)
+        if (valid[k]) hout[m - 1 - (r0 + k)] = (int32_t)h[k];
+    return true;
+}
+
+// Ticket t runs band group tasks[t].y (bands W*y .. W*y+W-1) of pair
+// tasks[t].x.  The only barriers are the three per task, which every wave
+// passes whatever its band did.
+template <int TM, int C, int W>
+__global__ __launch_bounds__(kWave * W) void comb_group_kernel(ChainArgs A)
+{
+    static_assert(W >= 2 && W <= 4, "group of 2..4 waves");
+    __shared__ uint32_t xfer[W - 1][kXferSlots][C];
+    __shared__ int32_t bring[kBRing][C];
+    __shared__ GroupLds group;
+    __shared__ unsigned s_task, s_stop;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = (int)(threadIdx.x >> 6);
+    lds_group *G = (lds_group *)&group;
+    for (;;) {
+        __syncthreads();  // every wave is done with the previous task's LDS
+        if (threadIdx.x == 0) {
+            const unsigned t = atomicAdd(A.ticket, 1u);
+            s_task = t;
+            s_stop = ((int)t >= A.ntasks || ld_flag(A.abort_flag)) ? 1u : 0u;
+            for (int i = 0; i < 4; ++i) {
+                G->prog[i] = -1;
+                G->cons[i] = -1;
+            }
+            G->stop = 0u;
+        }
+        __syncthreads();
+        const unsigned t = s_task, stop = s_stop;
+        __syncthreads();
+        if (stop) break;
+        const int2 task = A.tasks[t];
+        const PairDesc P = A.pairs[task.x];
+        const int g0 = task.y * W;
+        const int wact = min(W, P.nbands - g0);
+        const int trace_id = (int)t * W + w;
+        const bool traced = A.trace && trace_id < A.trace_cap && lane == 0;
+        if (traced) A.trace[2 * (size_t)trace_id] = __builtin_amdgcn_s_memrealtime();
+        if (w < wact) {
+            const int g = g0 + w;
+            // only a pair's last band has rows past m, and it is its group's last wave
+            const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
+            const int role = (w == 0 ? kFirst : 0) | (w == wact - 1 ? kLast : 0);
+            bool ok;
+            if (rowmask)
+                ok = role == (kFirst | kLast)
+                         ? group_band<TM, C, W, true, kFirst | kLast>(A, P, g, w, lane, xfer, bring, G)
+                         : group_band<TM, C, W, true, kLast>(A, P, g, w, lane, xfer, bring, G);
+            else if (role == (kFirst | kLast))
+                ok = group_band<TM, C, W, false, kFirst | kLast>(A, P, g, w, lane, xfer, bring, G);
+            else if (role == kFirst)
+                ok = group_band<TM, C, W, false, kFirst>(A, P, g, w, lane, xfer, bring, G);
+            else if (role == kLast)
+                ok = group_band<TM, C, W, false, kLast>(A, P, g, w, lane, xfer, bring, G);
+            else
+                ok = group_band<TM, C, W, false, 0>(A, P, g, w, lane, xfer, bring, G);
+            if (!ok && lane == 0) G->stop = 1u;  // let the group's other waves stop waiting
+        }
+        if (traced) A.trace[2 * (size_t)trace_id + 1] = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
 // Seeds the rings before a chain launch (blockIdx.y = pair): slot 0 holds
 // the grid's top edge (v_in[c], or the ids m+c) under tag 0 for band 0,
 // slot 1 holds a tag no band waits for.  A pair with no band (n == 0) gets
@@ -350,9 +626,17 @@ __global__ void comb_identity_kernel(int32_t *h_out, int32_t m, int32_t *v_out, 
 using namespace lcsk;
 
 template <int TM, int C>
-static void launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_threads, hipStream_t stream)
+static int launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_threads, hipStream_t stream)
 {
-    comb_chain_kernel<TM, C><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+    if (args.group_w == 1) {
+        comb_chain_kernel<TM, C><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+        return 0;
+    }
+    if (args.group_w == 4 && block_threads == 4 * kWave) {
+        comb_group_kernel<TM, C, 4><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+        return 0;
+    }
+    return -22;
 }
 
 int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs, int max_n, int tile_m,
@@ -364,15 +648,17 @@ int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs
     if (fill_blocks > 1024) fill_blocks = 1024;
     hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs, args.ticket, 2);
     if (args.ntasks > 0) {
+        int rc;
         switch (tile_m) {
-        case 1: launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
-        case 2: launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
-        case 4: launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
-        case 8: launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
-        case 16: launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
-        case 32: launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
-        default: return -22;
+        case 1: rc = launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
+        case 2: rc = launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
+        case 4: rc = launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
+        case 8: rc = launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
+        case 16: rc = launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
+        case 32: rc = launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
+        default: rc = -22;
         }
+        if (rc) return rc;
     }
     hipLaunchKernelGGL(comb_extract_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs);
     return hipGetLastError() == hipSuccess ? 0 : -5;

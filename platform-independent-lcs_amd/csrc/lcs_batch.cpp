@@ -14,9 +14,11 @@
 
 namespace lcs {
 
-// Tile heights a batch is grouped by (one persistent launch each).
+// A batch is grouped by tile height and variant (one persistent launch per
+// class): classes 0..5 register chain, 6..11 LDS groups.
 constexpr int kTileClasses[] = {1, 2, 4, 8, 16, 32};
-constexpr int kNumClasses = 6;
+constexpr int kNumTiles = 6;
+constexpr int kNumClasses = 2 * kNumTiles;
 
 struct Extras {
     hipStream_t gstream[kNumClasses] = {};
@@ -25,10 +27,10 @@ struct Extras {
     DevBuf batch_a, batch_b, batch_h, batch_v, batch_ring;
 };
 
-static int class_of(int tm)
+static int class_of(int tm, int group_w)
 {
-    for (int i = 0; i < kNumClasses; ++i)
-        if (kTileClasses[i] == tm) return i;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTileClasses[i] == tm) return group_w == 1 ? i : group_w == 4 ? kNumTiles + i : -1;
     return -1;
 }
 
@@ -158,7 +160,7 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     std::vector<int> wps(kNumClasses, 1);
     for (int32_t p = 0; p < npairs; ++p) {
         int tm = (scheds && scheds[p].tile_m) ? scheds[p].tile_m : default_batch_tile_m(m[p]);
-        const int c = class_of(tm);
+        const int c = class_of(tm, group_width(scheds ? scheds[p].variant : LCS_VARIANT_AUTO));
         if (c < 0) return LCS_EINVAL;
         if (scheds && scheds[p].waves_per_simd) {
             if (scheds[p].waves_per_simd < 1 || scheds[p].waves_per_simd > 8) return LCS_EINVAL;
@@ -183,7 +185,8 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
         auto &g = groups[(size_t)c];
         if (g.empty()) continue;
         PreparedLaunch L;
-        rc = prepare_pairs(ctx, g, kTileClasses[c], wps[(size_t)c], ring, X->gdescs[c], ctx->ctrl + 2 + 2 * c, L, s);
+        rc = prepare_pairs(ctx, g, kTileClasses[c % kNumTiles], wps[(size_t)c], ring, X->gdescs[c],
+                           ctx->ctrl + 2 + 2 * c, L, s, c < kNumTiles ? 1 : 4);
         if (rc) return rc;
         ring += pairs_ring_words(g);
         classes.push_back(c);

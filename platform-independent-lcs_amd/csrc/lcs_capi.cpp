@@ -105,10 +105,20 @@ size_t pairs_ring_words(const std::vector<PairDesc> &pairs)
     return w;
 }
 
-int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
-                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream)
+int group_width(int variant)
 {
-    if (!valid_tile_m(tm) || wps < 1 || wps > 8) return LCS_EINVAL;
+    switch (variant) {
+    case LCS_VARIANT_AUTO:
+    case LCS_VARIANT_CHAIN: return 1;
+    case LCS_VARIANT_CHAIN_LDS: return 4;
+    default: return 0;
+    }
+}
+
+int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
+                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w)
+{
+    if (!valid_tile_m(tm) || wps < 1 || wps > 8 || (group_w != 1 && group_w != 4)) return LCS_EINVAL;
     const int np = (int)pairs.size();
     L = PreparedLaunch{};
     if (np == 0) return LCS_OK;
@@ -124,10 +134,12 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
         max_bands = std::max(max_bands, P.nbands);
         max_n = std::max(max_n, std::max(P.n, P.m));
     }
+    // tasks: bands (or groups of group_w bands), band-major over the pairs
     std::vector<int2> tasks;
-    for (int32_t g = 0; g < max_bands; ++g)
+    const int32_t max_units = (max_bands + group_w - 1) / group_w;
+    for (int32_t g = 0; g < max_units; ++g)
         for (int p = 0; p < np; ++p)
-            if (g < pairs[(size_t)p].nbands) tasks.push_back(make_int2(p, g));
+            if (g < (pairs[(size_t)p].nbands + group_w - 1) / group_w) tasks.push_back(make_int2(p, g));
     const size_t pair_bytes = sizeof(PairDesc) * (size_t)np, task_bytes = sizeof(int2) * tasks.size();
     int rc = descs.reserve(ctx, pair_bytes + task_bytes + 256);
     if (rc) return rc;
@@ -144,9 +156,15 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     A.timeout_ticks = ctx->timeout_ticks;
     A.trace = ctx->band_trace;
     A.trace_cap = ctx->band_trace_cap;
+    A.group_w = group_w;
     const int waves_per_block = 4;
-    const long long waves = std::min<long long>((long long)tasks.size(), 4LL * ctx->num_cus * wps);
-    L.blocks = (int)std::max<long long>(1, (waves + waves_per_block - 1) / waves_per_block);
+    if (group_w == 1) {
+        const long long waves = std::min<long long>((long long)tasks.size(), 4LL * ctx->num_cus * wps);
+        L.blocks = (int)std::max<long long>(1, (waves + waves_per_block - 1) / waves_per_block);
+    } else {
+        // one group per workgroup, wps groups per CU
+        L.blocks = (int)std::max<long long>(1, std::min<long long>((long long)tasks.size(), (long long)ctx->num_cus * wps));
+    }
     L.threads = 64 * waves_per_block;
     L.npairs = np;
     L.max_n = max_n;
@@ -155,9 +173,9 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
 }
 
 int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, DevBuf &descs,
-                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream)
+                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w)
 {
-    return prepare_pairs(ctx, pairs, tm, wps, nullptr, descs, ctrl, L, stream);
+    return prepare_pairs(ctx, pairs, tm, wps, nullptr, descs, ctrl, L, stream, group_w);
 }
 
 // Prep, chain and extract kernels of a prepared launch (no copies or
@@ -170,13 +188,13 @@ int launch_prepared(const PreparedLaunch &L, hipStream_t stream)
 
 // Launches the band chain over `pairs` on `stream` with the context's ring,
 // descriptor buffer and ticket words.
-int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream)
+int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream, int group_w)
 {
     int rc = ctx->ring.reserve(ctx, pairs_ring_words(pairs) * sizeof(unsigned long long));
     if (rc) return rc;
     PreparedLaunch L;
     if ((rc = prepare_pairs(ctx, pairs, tm, wps, (unsigned long long *)ctx->ring.ptr, ctx->descs, ctx->ctrl, L,
-                            stream)))
+                            stream, group_w)))
         return rc;
     return launch_prepared(L, stream);
 }
@@ -197,7 +215,7 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 // launch, turn each into its permutation, merge them pairwise with the
 // steady ant (b-split, k = m), and read the strands off the product.
 int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n, int strips, int tm,
-                   int wps, int32_t *d_h, int32_t *d_v, hipStream_t stream)
+                   int wps, int group_w, int32_t *d_h, int32_t *d_v, hipStream_t stream)
 {
     const int32_t W = (n + strips - 1) / strips;
     const int S = (n + W - 1) / W;
@@ -226,7 +244,7 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
         size[(size_t)s] = (int64_t)m + P.n;
         acc += size[(size_t)s];
     }
-    if ((rc = enqueue_pairs(ctx, pairs, tm, wps, stream))) return rc;
+    if ((rc = enqueue_pairs(ctx, pairs, tm, wps, stream, group_w))) return rc;
     // enqueue_pairs staged the descriptors into ctx->descs; reuse them for the permutations
     if ((rc = upload(ctx, d_off, off.data(), off.size() * sizeof(int64_t), stream))) return rc;
     if ((rc = lcsk_launch_from_strands((const PairDesc *)ctx->descs.ptr, S, m + W, permA, d_off, stream))) return rc;
@@ -282,6 +300,8 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     const int wps = (sched && sched->waves_per_simd) ? sched->waves_per_simd : 1;
     if (wps < 1 || wps > 8) return LCS_EINVAL;
     if (strips_req < 0) return LCS_EINVAL;
+    const int group_w = group_width(sched ? sched->variant : LCS_VARIANT_AUTO);
+    if (!group_w) return LCS_EINVAL;
     if (!tm) {
         tm = default_tile_m(ctx, m);
         if (strips_req != 1 && !d_h_in && !d_v_in && tm < 4 && default_strips(ctx, m, n, 4) > 1) tm = 4;
@@ -290,7 +310,7 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     if (strips > 1 && (d_h_in || d_v_in)) return LCS_EINVAL;  // fixups use explicit strands: one strip
     if (strips > 1 && m > kMaxCoreK) return LCS_ERANGE;
     if (strips > n) strips = n;
-    if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, d_h, d_v, stream);
+    if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, group_w, d_h, d_v, stream);
     std::vector<PairDesc> pairs(1);
     PairDesc &P = pairs[0];
     P = PairDesc{};
@@ -302,7 +322,7 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     P.v_out = d_v;
     P.m = m;
     P.n = n;
-    return enqueue_pairs(ctx, pairs, tm, wps, stream);
+    return enqueue_pairs(ctx, pairs, tm, wps, stream, group_w);
 }
 
 int check_abort(lcs_ctx *ctx)

@@ -58,12 +58,15 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
 int check_abort(lcs_ctx *ctx);
 int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream);
 size_t pairs_ring_words(const std::vector<PairDesc> &pairs);
+// group_w: 1 = register chain (one wave per band), 4 = LDS groups of 4 bands
 int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
-                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream);
+                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w = 1);
+// group width of a schedule variant (LCS_VARIANT_*), 0 if unknown
+int group_width(int variant);
 int launch_prepared(const PreparedLaunch &L, hipStream_t stream);
 int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, DevBuf &descs,
-                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream);
-int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream);
+                            unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w = 1);
+int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream, int group_w = 1);
 int default_batch_tile_m(int32_t m);
 bool valid_tile_m(int tm);
 int32_t bands_of(int32_t m, int32_t n, int tm);

@@ -38,3 +38,12 @@ def test_batch_equals_single_calls(ctx):
     for p, h, v in zip(pairs, hs, vs):
         h1, v1, _ = ctx.comb(p.a, p.b)
         assert np.array_equal(h, h1) and np.array_equal(v, v1)
+
+
+def test_batch_mixed_variants(ctx):
+    pairs = lcs_amd.workload_batch(40, seed=21, lo=1, hi=5000)
+    scheds = [lcs_amd.schedule(tile_m=[1, 2, 4, 8][i % 4], variant=1 + (i % 2)) for i in range(len(pairs))]
+    hs, vs = ctx.batch_comb([p.a for p in pairs], [p.b for p in pairs], scheds=scheds)
+    for p, h, v in zip(pairs, hs, vs):
+        ho, vo = oracle.comb(p.a, p.b)
+        assert np.array_equal(h, ho) and np.array_equal(v, vo)

@@ -119,3 +119,47 @@ def test_thin_pair_default_schedule(ctx):
     h, v, _ = ctx.comb(a, b)
     ho, vo = oracle.comb(a, b, order="mt", threads=8)
     assert np.array_equal(h, ho) and np.array_equal(v, vo)
+
+
+LDS = 2  # LCS_VARIANT_CHAIN_LDS
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 100), (100, 1), (63, 64), (65, 129), (300, 17), (17, 300),
+                                 (64 * 4 * 4, 500), (64 * 4 * 4 + 1, 500), (64 * 5, 1000)])
+def test_lds_groups_edge_shapes(ctx, m, n):
+    # groups of 4 bands: full groups, partial last groups (1..3 bands) and
+    # a partial last band
+    rng = np.random.default_rng(m * 7 + n)
+    a = rng.integers(0, 3, m).astype(np.int32)
+    b = rng.integers(0, 3, n).astype(np.int32)
+    for tm in (1, 4):
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=LDS))
+
+
+@pytest.mark.parametrize("tm", [1, 2, 4, 8, 16, 32])
+def test_lds_groups_tile_heights(ctx, tm):
+    rng = np.random.default_rng(100 + tm)
+    for m, n in [(1000, 1500), (4096 * 3 + 11, 777), (5000, 64 * 32 + 5)]:
+        a = rng.integers(0, 4, m).astype(np.int32)
+        b = rng.integers(0, 4, n).astype(np.int32)
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=LDS))
+
+
+def test_lds_groups_reference_sizes_and_strips(ctx):
+    a = lcs_amd.gen_binary(888, 7330)
+    b = lcs_amd.gen_binary(999, 9944)
+    _check(ctx, a, b, lcs_amd.schedule(variant=LDS))
+    _check(ctx, a, b, lcs_amd.schedule(tile_m=2, waves_per_simd=2, variant=LDS))
+    rng = np.random.default_rng(9)
+    a = rng.integers(0, 4, 3000).astype(np.int32)
+    b = rng.integers(0, 4, 40000).astype(np.int32)
+    _check(ctx, a, b, lcs_amd.schedule(tile_m=4, strips=4, variant=LDS))
+    w = lcs_amd.workload_single("mutated2^14")
+    r1 = ctx.comb(w.a, w.b)
+    r2 = ctx.comb(w.a, w.b, lcs_amd.schedule(tile_m=4, variant=LDS))
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+
+
+def test_unknown_variant_is_rejected(ctx):
+    with pytest.raises(lcs_amd.LcsError):
+        ctx.comb(np.zeros(10, np.int32), np.zeros(10, np.int32), lcs_amd.schedule(variant=7))
