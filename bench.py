@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="cross-check two schedules bit-for-bit")
+    ap.add_argument("--batch-pairs", type=int, default=10000, help="pairs of the batched leg (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=600, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-rate", type=float, default=150.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device-to-device copies of 256 MiB (HBM counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -80,8 +85,155 @@ def read_traffic(path: str):
     return None
 
 
+def calibrate_copy():
+    """Three 256 MiB device-to-device copies: the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the comb kernel's counters."""
+    import ctypes
+    h = H.hip()
+    n = 64 << 20
+    src, dst = H.DeviceArray(n), H.DeviceArray(n)
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    for _ in range(3):
+        assert h.hipMemcpy(dst.ptr, src.ptr, n * 4, 3) == 0   # hipMemcpyDeviceToDevice
+    H.synchronize()
+    print(json.dumps({"copied_bytes": n * 4, "copies": 3}), flush=True)
+
+
+def learned_model(ctx, mode: int):
+    """The cost model trained on the GPU on the autotuner's committed timing
+    table of this device (None when no table is there)."""
+    from lcs_amd import tuning
+    if not os.path.exists(tuning.TABLES[mode]):
+        return None, None
+    table = tuning.load_table(mode)
+    t0 = time.perf_counter()
+    mlp = tuning.train(ctx, table)
+    return mlp, {"table": os.path.relpath(tuning.TABLES[mode], ROOT), "shapes": len(table["rows"]),
+                 "train_s": round(time.perf_counter() - t0, 2)}
+
+
+def bench_batch(ctx, npairs: int):
+    """10,000 pairs, m and n log-uniform in [256, 16384], alphabet 4, B = A
+    mutated at a per-pair rate in [5%, 20%] (20% of it indels), resident in
+    HBM.  Plain path: one lcs_comb_device call per pair (the drop-in's own
+    schedule).  Batched path: lcs_batch_comb_device, every pair in one
+    hipGraph launch, schedules from the built-in heuristic and from the
+    learned cost model.  Every batched result is compared with the plain
+    path's, pair by pair."""
+    from lcs_amd import tuning
+    L = lcs_amd.lib()
+    t0 = time.perf_counter()
+    pairs = lcs_amd.workload_batch(npairs, seed=7)
+    lay = lcs_amd.BatchLayout([p.a for p in pairs], [p.b for p in pairs])
+    gen_s = time.perf_counter() - t0
+    cells = float(np.sum(lay.m.astype(np.float64) * lay.n))
+    da, db = H.DeviceArray.from_numpy(lay.a_all), H.DeviceArray.from_numpy(lay.b_all)
+    dh0, dv0 = H.DeviceArray(lay.a_all.size), H.DeviceArray(lay.b_all.size)
+    dh1, dv1 = H.DeviceArray(lay.a_all.size), H.DeviceArray(lay.b_all.size)
+
+    def plain():
+        for p in range(npairs):
+            ao, bo = int(lay.a_off[p]) * 4, int(lay.b_off[p]) * 4
+            rc = L.lcs_comb_device(ctx.handle, da.ptr.value + ao, int(lay.m[p]), db.ptr.value + bo, int(lay.n[p]),
+                                   None, dh0.ptr.value + ao, dv0.ptr.value + bo, None)
+            if rc:
+                raise lcs_amd.LcsError("lcs_comb_device", rc)
+        H.synchronize()
+
+    plain()   # warm-up
+    t0 = time.perf_counter()
+    plain()
+    plain_s = time.perf_counter() - t0
+    ref_h, ref_v = dh0.numpy(), dv0.numpy()
+
+    def batched(scheds):
+        best, kms = 1e30, 0.0
+        for _ in range(3):
+            t1 = time.perf_counter()
+            lcs_amd.batch_comb_device(ctx, lay, da.ptr, db.ptr, dh1.ptr, dv1.ptr, scheds)
+            dt = time.perf_counter() - t1
+            if dt < best:
+                best, kms = dt, ctx.info()[1]
+        exact = bool(np.array_equal(dh1.numpy(), ref_h) and np.array_equal(dv1.numpy(), ref_v))
+        return best, kms, exact
+
+    heur_s, heur_kms, heur_exact = batched(None)
+    out = {"workload": f"{npairs} pairs, m,n log-uniform [256,16384], alphabet 4, 5-20% mutations",
+           "cells": cells, "gen_s": round(gen_s, 1),
+           "plain_pairs_per_s": round(npairs / plain_s, 1), "plain_s": round(plain_s, 3),
+           "batched_heuristic_pairs_per_s": round(npairs / heur_s, 1), "batched_heuristic_s": round(heur_s, 4),
+           "batched_heuristic_kernel_ms": round(heur_kms, 3), "batched_heuristic_bit_exact": heur_exact}
+    mlp, info = learned_model(ctx, 1)
+    if mlp is not None:
+        scheds, idx = tuning.choose_schedules(mlp, lay.m, lay.n, [p.alphabet for p in pairs],
+                                              [p.rate for p in pairs])
+        l_s, l_kms, l_exact = batched(scheds)
+        out.update({"batched_learned_pairs_per_s": round(npairs / l_s, 1), "batched_learned_s": round(l_s, 4),
+                    "batched_learned_kernel_ms": round(l_kms, 3), "batched_learned_bit_exact": l_exact,
+                    "learned_choice_histogram": {f"tm{tuning.CANDIDATES[i].tile_m}v{tuning.CANDIDATES[i].variant}":
+                                                 int(c) for i, c in zip(*np.unique(idx, return_counts=True))},
+                    "cost_model": info})
+        mlp.close()
+        best_s = min(heur_s, l_s)
+    else:
+        out["cost_model"] = "no autotune table: heuristic schedules only"
+        best_s = heur_s
+    out["speedup_vs_plain"] = round(plain_s / best_s, 2)
+    out["batched_gcups"] = round(cells / best_s / 1e9, 1)
+    return out
+
+
+def bench_serve(ctx, nreq: int, rate: float):
+    """Poisson arrivals of pair comparisons, lengths log-uniform in
+    [1024, 262144], alphabet 4 or 20 (50/50), B from A by the same mutation
+    model; admitted by an HBM budget into overlapping micro-batches, the
+    schedule of each request chosen by the learned cost model.  Latency is
+    submit-to-result on the host."""
+    from lcs_amd import tuning
+    pairs = lcs_amd.workload_batch(nreq, seed=9, lo=1024, hi=262144, alphabet=(4, 20))
+    srv = lcs_amd.Server(ctx, hbm_budget=8 << 30, max_inflight=4)
+    mlp, info = learned_model(ctx, 0)
+    if mlp is not None:
+        srv.set_cost_model(mlp, tuning.CANDIDATES)
+    feats = tuning.features([p.a.size for p in pairs], [p.b.size for p in pairs], [p.alphabet for p in pairs],
+                            [p.rate for p in pairs])
+    rng = np.random.default_rng(5)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    ids, t_sub = [], []
+    t0 = time.perf_counter()
+    next_t = t0
+    for p, f, gap in zip(pairs, feats, gaps):
+        next_t += gap
+        while time.perf_counter() < next_t:
+            time.sleep(max(0.0, min(next_t - time.perf_counter(), 0.002)))
+        ids.append(srv.submit(p.a, p.b, None, f))
+    outs = [srv.wait(i, timeout_ms=120000) for i in ids]
+    wall = time.perf_counter() - t0
+    lat = np.array([srv.times(i)[1] for i in ids]) / 1e3   # ms
+    st = srv.stats()
+    # a sample of results against the same pair alone (bit-exact)
+    exact = True
+    for p, o in list(zip(pairs, outs))[:: max(1, nreq // 12)]:
+        h1, v1, _ = ctx.comb(p.a, p.b)
+        exact = exact and np.array_equal(o[0], h1) and np.array_equal(o[1], v1)
+    for i in ids:
+        srv.release(i)
+    srv.close()
+    if mlp is not None:
+        mlp.close()
+    cells = float(sum(p.a.size * p.b.size for p in pairs))
+    return {"workload": f"{nreq} requests, Poisson {rate}/s, lengths log-uniform [1024,262144], alphabet 4|20",
+            "pairs_per_s": round(nreq / wall, 1), "gcups": round(cells / wall / 1e9, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "micro_batches": st["batches"], "max_inflight": st["max_inflight"],
+            "bit_exact_vs_alone_sample": bool(exact), "cost_model": info or "none (heuristic schedules)"}
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU sharding is a design only)", file=sys.stderr)
@@ -152,6 +304,10 @@ def main():
     }
     if checked is not None:
         out["cross_schedule_bit_exact"] = checked
+    if args.batch_pairs > 0:
+        out["batched"] = bench_batch(ctx, args.batch_pairs)
+    if args.serve_requests > 0:
+        out["served"] = bench_serve(ctx, args.serve_requests, args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a, b, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -145,6 +145,14 @@ int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs,
                    const int32_t *b_all, const int64_t *b_off, const int32_t *n,
                    const lcs_schedule *scheds, int32_t use_graph,
                    int32_t *h_out, int32_t *v_out);
+/* The same with the symbols and strands in device memory (d_*: device
+ * pointers; offsets, lengths and schedules stay host arrays).  Synchronous;
+ * lcs_ctx_info reports the launch's kernel time. */
+int lcs_batch_comb_device(lcs_ctx *ctx, int32_t npairs,
+                          const int32_t *d_a_all, const int64_t *a_off, const int32_t *m,
+                          const int32_t *d_b_all, const int64_t *b_off, const int32_t *n,
+                          const lcs_schedule *scheds, int32_t use_graph,
+                          int32_t *d_h_out, int32_t *d_v_out);
 
 /* ---- autotuner ------------------------------------------------------------
  * No counterpart in the reference (its variants were swept offline by
@@ -152,7 +160,8 @@ int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs,
  * synthetic pairs of the shape (A uniform over `alphabet`, B = A mutated at
  * `mutation_rate`), best of `reps` runs after a warm-up: mode 0 times one
  * pair through lcs_comb_device (cost = kernel ms); mode 1 times a batch of
- * identical-shape pairs through lcs_batch_comb (cost = kernel ms per pair).
+ * K identical-shape pairs through lcs_batch_comb (cost = kernel ms per pair,
+ * K = 2^28 / (m*n) clamped to [8, 4096]).
  * cost_ms has ncand entries. */
 int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t alphabet, float mutation_rate, uint64_t seed,
                          const lcs_schedule *candidates, int32_t ncand, int32_t mode, int32_t reps, float *cost_ms);

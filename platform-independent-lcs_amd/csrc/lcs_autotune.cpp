@@ -23,7 +23,8 @@ extern "C" int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t 
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
     // pairs: A uniform, B = A mutated (substitutions and indels 4:1)
     const double cells = (double)m * (double)n;
-    const int K = mode == 0 ? 1 : (int)std::min(4096.0, std::max(64.0, 8589934592.0 / cells));
+    // mode 1: about 2^28 cells per timed launch (at least 8 pairs, at most 4096)
+    const int K = mode == 0 ? 1 : (int)std::min(4096.0, std::max(8.0, 268435456.0 / cells));
     std::vector<int32_t> a_all((size_t)K * m), b_all((size_t)K * n);
     std::vector<int64_t> a_off((size_t)K), b_off((size_t)K);
     std::vector<int32_t> ms((size_t)K, m), ns((size_t)K, n);

@@ -127,35 +127,17 @@ int run_launches(lcs_ctx *ctx, Extras *X, const std::vector<int> &classes,
 
 using namespace lcs;
 
-extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all, const int64_t *a_off,
-                              const int32_t *m, const int32_t *b_all, const int64_t *b_off, const int32_t *n,
-                              const lcs_schedule *scheds, int32_t use_graph, int32_t *h_out, int32_t *v_out)
+namespace {
+
+// Groups the pairs by (tile height, variant), prepares one persistent launch
+// per class and runs them (captured in a hipGraph when use_graph).  All
+// arrays of symbols and strands are device pointers; offsets, lengths and
+// schedules are host arrays.
+int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const int64_t *a_off, const int32_t *m,
+               const int32_t *db, const int64_t *b_off, const int32_t *n, const lcs_schedule *scheds,
+               int32_t use_graph, int32_t *dh, int32_t *dv)
 {
-    if (!ctx || npairs < 0) return LCS_EINVAL;
-    if (npairs == 0) return LCS_OK;
-    if (!a_off || !m || !b_off || !n) return LCS_EINVAL;
-    int64_t total_a = 0, total_b = 0;
-    for (int32_t p = 0; p < npairs; ++p) {
-        if (m[p] < 0 || n[p] < 0 || a_off[p] < 0 || b_off[p] < 0) return LCS_EINVAL;
-        if ((long long)m[p] + n[p] > 0x7fffffffLL) return LCS_ERANGE;
-        total_a = std::max<int64_t>(total_a, a_off[p] + m[p]);
-        total_b = std::max<int64_t>(total_b, b_off[p] + n[p]);
-    }
-    if ((total_a && (!a_all || !h_out)) || (total_b && (!b_all || !v_out))) return LCS_EINVAL;
-    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
-    Extras *X = get_extras(ctx);
-    if (!X) return LCS_ENOMEM;
-    int rc;
-    if ((rc = X->batch_a.reserve(ctx, (size_t)std::max<int64_t>(total_a, 1) * 4))) return rc;
-    if ((rc = X->batch_b.reserve(ctx, (size_t)std::max<int64_t>(total_b, 1) * 4))) return rc;
-    if ((rc = X->batch_h.reserve(ctx, (size_t)std::max<int64_t>(total_a, 1) * 4))) return rc;
-    if ((rc = X->batch_v.reserve(ctx, (size_t)std::max<int64_t>(total_b, 1) * 4))) return rc;
-    int32_t *da = (int32_t *)X->batch_a.ptr, *db = (int32_t *)X->batch_b.ptr;
-    int32_t *dh = (int32_t *)X->batch_h.ptr, *dv = (int32_t *)X->batch_v.ptr;
     hipStream_t s = ctx->stream;
-    if (total_a && hipMemcpyAsync(da, a_all, (size_t)total_a * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
-    if (total_b && hipMemcpyAsync(db, b_all, (size_t)total_b * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
-    // group by tile height
     std::vector<std::vector<PairDesc>> groups(kNumClasses);
     std::vector<int> wps(kNumClasses, 1);
     for (int32_t p = 0; p < npairs; ++p) {
@@ -177,6 +159,7 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     }
     size_t ring_words = 0;
     for (auto &g : groups) ring_words += pairs_ring_words(g);
+    int rc;
     if ((rc = X->batch_ring.reserve(ctx, std::max<size_t>(ring_words, 1) * 8))) return rc;
     unsigned long long *ring = (unsigned long long *)X->batch_ring.ptr;
     std::vector<int> classes;
@@ -194,8 +177,6 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     }
     if (hipStreamSynchronize(s) != hipSuccess) return LCS_EIO;
     if ((rc = run_launches(ctx, X, classes, launches, use_graph != 0))) return rc;
-    if (total_a && hipMemcpyAsync(h_out, dh, (size_t)total_a * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
-    if (total_b && hipMemcpyAsync(v_out, dv, (size_t)total_b * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
     if (hipStreamSynchronize(s) != hipSuccess) return LCS_EIO;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_ms = ms;
@@ -204,4 +185,66 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     for (int c : classes)
         if (flags[2 + 2 * c + 1]) return LCS_ETIMEDOUT;
     return LCS_OK;
+}
+
+int check_batch_args(int32_t npairs, const int64_t *a_off, const int32_t *m, const int64_t *b_off,
+                     const int32_t *n, int64_t &total_a, int64_t &total_b)
+{
+    if (!a_off || !m || !b_off || !n) return LCS_EINVAL;
+    total_a = total_b = 0;
+    for (int32_t p = 0; p < npairs; ++p) {
+        if (m[p] < 0 || n[p] < 0 || a_off[p] < 0 || b_off[p] < 0) return LCS_EINVAL;
+        if ((long long)m[p] + n[p] > 0x7fffffffLL) return LCS_ERANGE;
+        total_a = std::max<int64_t>(total_a, a_off[p] + m[p]);
+        total_b = std::max<int64_t>(total_b, b_off[p] + n[p]);
+    }
+    return LCS_OK;
+}
+
+}  // namespace
+
+extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all, const int64_t *a_off,
+                              const int32_t *m, const int32_t *b_all, const int64_t *b_off, const int32_t *n,
+                              const lcs_schedule *scheds, int32_t use_graph, int32_t *h_out, int32_t *v_out)
+{
+    if (!ctx || npairs < 0) return LCS_EINVAL;
+    if (npairs == 0) return LCS_OK;
+    int64_t total_a = 0, total_b = 0;
+    int rc = check_batch_args(npairs, a_off, m, b_off, n, total_a, total_b);
+    if (rc) return rc;
+    if ((total_a && (!a_all || !h_out)) || (total_b && (!b_all || !v_out))) return LCS_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    Extras *X = get_extras(ctx);
+    if (!X) return LCS_ENOMEM;
+    if ((rc = X->batch_a.reserve(ctx, (size_t)std::max<int64_t>(total_a, 1) * 4))) return rc;
+    if ((rc = X->batch_b.reserve(ctx, (size_t)std::max<int64_t>(total_b, 1) * 4))) return rc;
+    if ((rc = X->batch_h.reserve(ctx, (size_t)std::max<int64_t>(total_a, 1) * 4))) return rc;
+    if ((rc = X->batch_v.reserve(ctx, (size_t)std::max<int64_t>(total_b, 1) * 4))) return rc;
+    int32_t *da = (int32_t *)X->batch_a.ptr, *db = (int32_t *)X->batch_b.ptr;
+    int32_t *dh = (int32_t *)X->batch_h.ptr, *dv = (int32_t *)X->batch_v.ptr;
+    hipStream_t s = ctx->stream;
+    if (total_a && hipMemcpyAsync(da, a_all, (size_t)total_a * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
+    if (total_b && hipMemcpyAsync(db, b_all, (size_t)total_b * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
+    if ((rc = batch_core(ctx, X, npairs, da, a_off, m, db, b_off, n, scheds, use_graph, dh, dv))) return rc;
+    if (total_a && hipMemcpyAsync(h_out, dh, (size_t)total_a * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
+    if (total_b && hipMemcpyAsync(v_out, dv, (size_t)total_b * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
+    if (hipStreamSynchronize(s) != hipSuccess) return LCS_EIO;
+    return LCS_OK;
+}
+
+extern "C" int lcs_batch_comb_device(lcs_ctx *ctx, int32_t npairs, const int32_t *d_a_all, const int64_t *a_off,
+                                     const int32_t *m, const int32_t *d_b_all, const int64_t *b_off,
+                                     const int32_t *n, const lcs_schedule *scheds, int32_t use_graph,
+                                     int32_t *d_h_out, int32_t *d_v_out)
+{
+    if (!ctx || npairs < 0) return LCS_EINVAL;
+    if (npairs == 0) return LCS_OK;
+    int64_t total_a = 0, total_b = 0;
+    int rc = check_batch_args(npairs, a_off, m, b_off, n, total_a, total_b);
+    if (rc) return rc;
+    if ((total_a && (!d_a_all || !d_h_out)) || (total_b && (!d_b_all || !d_v_out))) return LCS_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    Extras *X = get_extras(ctx);
+    if (!X) return LCS_ENOMEM;
+    return batch_core(ctx, X, npairs, d_a_all, a_off, m, d_b_all, b_off, n, scheds, use_graph, d_h_out, d_v_out);
 }

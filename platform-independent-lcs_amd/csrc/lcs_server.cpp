@@ -34,14 +34,16 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int kClasses = 6;
-constexpr int kTiles[kClasses] = {1, 2, 4, 8, 16, 32};
+// launch classes: tile height x variant (0..5 register chain, 6..11 LDS groups)
+constexpr int kNumTiles = 6;
+constexpr int kClasses = 2 * kNumTiles;
+constexpr int kTiles[kNumTiles] = {1, 2, 4, 8, 16, 32};
 constexpr size_t kAlign = 256;
 
-int class_index(int tm)
+int class_index(int tm, int group_w)
 {
-    for (int i = 0; i < kClasses; ++i)
-        if (kTiles[i] == tm) return i;
+    for (int i = 0; i < kNumTiles; ++i)
+        if (kTiles[i] == tm) return group_w == 1 ? i : group_w == 4 ? kNumTiles + i : -1;
     return -1;
 }
 
@@ -179,7 +181,7 @@ int launch_slot(lcs_server *S, Slot &slot)
         if (n && hipMemcpyAsync(db, R->b.data(), (size_t)n * 4, hipMemcpyHostToDevice, slot.stream) != hipSuccess)
             return LCS_EIO;
         const int tm = R->sched.tile_m ? R->sched.tile_m : default_batch_tile_m(m);
-        const int c = class_index(tm);
+        const int c = class_index(tm, group_width(R->sched.variant));
         if (c < 0) return LCS_EINVAL;
         PairDesc P{};
         P.a = da;
@@ -197,8 +199,9 @@ int launch_slot(lcs_server *S, Slot &slot)
         if (g.empty()) continue;
         // each pair's ring lies inside its request's arena block
         PreparedLaunch L;
-        int rc = prepare_pairs_own_rings(ctx, g, kTiles[c], 1, slot.descs[c],
-                                         S->ctrl + 2 * (slot_index * kClasses + c), L, slot.stream);
+        int rc = prepare_pairs_own_rings(ctx, g, kTiles[c % kNumTiles], 1, slot.descs[c],
+                                         S->ctrl + 2 * (slot_index * kClasses + c), L, slot.stream,
+                                         c < kNumTiles ? 1 : 4);
         if (rc) return rc;
         if ((rc = launch_prepared(L, slot.stream))) return rc;
     }
@@ -366,6 +369,10 @@ int lcs_server_destroy(lcs_server *S)
 int lcs_server_set_cost_model(lcs_server *S, lcs_mlp *model, const lcs_schedule *candidates, int32_t ncand)
 {
     if (!S || ncand < 0 || (ncand && !candidates)) return LCS_EINVAL;
+    for (int32_t i = 0; i < ncand; ++i) {
+        const int gw = group_width(candidates[i].variant);
+        if (!gw || !candidates[i].tile_m || class_index(candidates[i].tile_m, gw) < 0) return LCS_EINVAL;
+    }
     std::lock_guard<std::mutex> lk(S->mu);
     S->model = model;
     S->candidates.assign(candidates, candidates + ncand);
@@ -377,7 +384,9 @@ int lcs_server_submit(lcs_server *S, const int32_t *a, int32_t m, const int32_t 
 {
     if (!S || !request_id || m < 0 || n < 0 || (m && !a) || (n && !b)) return LCS_EINVAL;
     if ((long long)m + n > 0x7fffffffLL) return LCS_ERANGE;
-    if (sched && sched->tile_m && class_index(sched->tile_m) < 0) return LCS_EINVAL;
+    if (sched && (group_width(sched->variant) == 0 ||
+                  (sched->tile_m && class_index(sched->tile_m, group_width(sched->variant)) < 0)))
+        return LCS_EINVAL;
     auto R = std::make_shared<Request>();
     R->a.assign(a, a + m);
     R->b.assign(b, b + n);

@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
     L.lcs_permutation_hash.restype = i64
     L.lcs_braid_multiply.argtypes = [vp, _i32p, i32, _i32p, i32, i32, i32, _i32p]
     L.lcs_batch_comb.argtypes = [vp, i32, _i32p, _i64p, _i32p, _i32p, _i64p, _i32p, sched_p, i32, _i32p, _i32p]
+    L.lcs_batch_comb_device.argtypes = [vp, i32, vp, _i64p, _i32p, vp, _i64p, _i32p, sched_p, i32, vp, vp]
     L.lcs_debug_band_trace.argtypes = [vp, i32, vp]
     f32p = ctypes.POINTER(ctypes.c_float)
     L.lcs_mlp_create.argtypes = [vp, i32, i32, i32, ctypes.c_uint64, ctypes.POINTER(vp)]
@@ -239,6 +240,38 @@ class Context:
         _check("lcs_batch_comb", rc)
         return ([h[a_off[p]: a_off[p] + m[p]].copy() for p in range(npairs)],
                 [v[b_off[p]: b_off[p] + n[p]].copy() for p in range(npairs)])
+
+
+class BatchLayout:
+    """Concatenated pairs: a_all / b_all with per-pair offsets and lengths
+    (the layout lcs_batch_comb and lcs_batch_comb_device take)."""
+
+    def __init__(self, a_list, b_list):
+        self.npairs = len(a_list)
+        self.m = np.array([len(x) for x in a_list], np.int32)
+        self.n = np.array([len(x) for x in b_list], np.int32)
+        self.a_off = np.zeros(self.npairs, np.int64)
+        self.b_off = np.zeros(self.npairs, np.int64)
+        if self.npairs:
+            self.a_off[1:] = np.cumsum(self.m[:-1], dtype=np.int64)
+            self.b_off[1:] = np.cumsum(self.n[:-1], dtype=np.int64)
+        self.a_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in a_list]) if self.npairs else [])
+        self.b_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in b_list]) if self.npairs else [])
+
+    def split(self, h_all, v_all):
+        return ([h_all[self.a_off[p]: self.a_off[p] + self.m[p]] for p in range(self.npairs)],
+                [v_all[self.b_off[p]: self.b_off[p] + self.n[p]] for p in range(self.npairs)])
+
+
+def batch_comb_device(ctx: "Context", layout: BatchLayout, d_a, d_b, d_h, d_v, scheds=None,
+                      use_graph: bool = True) -> None:
+    """lcs_batch_comb_device: d_a, d_b, d_h, d_v are device addresses of the
+    concatenated symbols and strands (layout's offsets)."""
+    sarr = (Schedule * layout.npairs)(*scheds) if scheds is not None else None
+    rc = lib().lcs_batch_comb_device(ctx.handle, layout.npairs, d_a, layout.a_off.ctypes.data_as(_i64p),
+                                     _p(layout.m), d_b, layout.b_off.ctypes.data_as(_i64p), _p(layout.n),
+                                     sarr, int(use_graph), d_h, d_v)
+    _check("lcs_batch_comb_device", rc)
 
 
 def _f(x: np.ndarray):
