@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="cross-check two schedules bit-for-bit")
+    ap.add_argument("--no-zipf", action="store_true", help="skip the m=2^14, n=2^22 Zipf leg")
     ap.add_argument("--batch-pairs", type=int, default=10000, help="pairs of the batched leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=150.0, help="Poisson arrival rate (requests/s)")
@@ -183,6 +184,43 @@ def bench_batch(ctx, npairs: int):
     return out
 
 
+def bench_zipf(ctx, steps: int = 3):
+    """m = 2^14, n = 2^22, alphabet 26 with Zipf(1.1) frequencies, 64 planted
+    5%-mutated copies of A in B.  The default schedule cuts b into column
+    strips, combs them in one chain launch and merges the strip permutations
+    with the GPU steady ant (k = m); the result is compared bit for bit with
+    the same pair combed as one strip."""
+    L = lcs_amd.lib()
+    w = lcs_amd.workload_single("zipf_planted", seed=1)
+    m, n = w.a.size, w.b.size
+    da, db = H.DeviceArray.from_numpy(w.a), H.DeviceArray.from_numpy(w.b)
+    dh, dv = H.DeviceArray(m), H.DeviceArray(n)
+
+    def run(sched):
+        rc = L.lcs_comb_device(ctx.handle, da.ptr, m, db.ptr, n, sched, dh.ptr, dv.ptr, None)
+        if rc:
+            raise lcs_amd.LcsError("lcs_comb_device", rc)
+        H.synchronize()
+
+    run(None)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(None)
+    dt = (time.perf_counter() - t0) / steps
+    h, v = dh.numpy(), dv.numpy()
+    one = lcs_amd.schedule(strips=1)
+    run(one)
+    t0 = time.perf_counter()
+    run(one)
+    dt1 = time.perf_counter() - t0
+    exact = bool(np.array_equal(h, dh.numpy()) and np.array_equal(v, dv.numpy()))
+    cells = float(m) * n
+    return {"workload": "m=2^14, n=2^22, alphabet 26 Zipf(1.1), 64 planted copies (5% subs)",
+            "gcups": round(cells / dt / 1e9, 1), "ms": round(dt * 1e3, 2),
+            "one_strip_gcups": round(cells / dt1 / 1e9, 1), "strips_vs_one_strip_bit_exact": exact,
+            "llcs": int((v < m).sum())}
+
+
 def bench_serve(ctx, nreq: int, rate: float):
     """Poisson arrivals of pair comparisons, lengths log-uniform in
     [1024, 262144], alphabet 4 or 20 (50/50), B from A by the same mutation
@@ -304,6 +342,8 @@ def main():
     }
     if checked is not None:
         out["cross_schedule_bit_exact"] = checked
+    if not args.no_zipf:
+        out["zipf_planted"] = bench_zipf(ctx)
     if args.batch_pairs > 0:
         out["batched"] = bench_batch(ctx, args.batch_pairs)
     if args.serve_requests > 0:

@@ -159,9 +159,11 @@ int lcs_batch_comb_device(lcs_ctx *ctx, int32_t npairs,
  * notebooks, tests/test_prepare.ipynb).  Times each candidate schedule on
  * synthetic pairs of the shape (A uniform over `alphabet`, B = A mutated at
  * `mutation_rate`), best of `reps` runs after a warm-up: mode 0 times one
- * pair through lcs_comb_device (cost = kernel ms); mode 1 times a batch of
- * K identical-shape pairs through lcs_batch_comb (cost = kernel ms per pair,
- * K = 2^28 / (m*n) clamped to [8, 4096]).
+ * pair through lcs_comb_device (cost = kernel ms); mode 1 prices a pair in
+ * a batch that keeps the GPU busy: K identical-shape pairs (enough for two
+ * bands of 64*32 rows per SIMD, K in [8, 4096]) through lcs_batch_comb,
+ * columns cut to about 2^30 / (K*m) (at least 1024), cost = kernel ms per
+ * pair scaled by (n + 64) / (columns + 64).
  * cost_ms has ncand entries. */
 int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t alphabet, float mutation_rate, uint64_t seed,
                          const lcs_schedule *candidates, int32_t ncand, int32_t mode, int32_t reps, float *cost_ms);

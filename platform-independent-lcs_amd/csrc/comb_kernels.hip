@@ -218,17 +218,22 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
-            // The band above is not that far yet: poll.
+            // The band above is not that far yet: poll.  Each round re-reads
+            // the missing words first; the abort flag and the clock (one more
+            // round trip each) are looked at every 8th round only.
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-            do {
-                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
-                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
-                    raise_abort(A.abort_flag);
-                    return false;
-                }
+            for (int round = 1;; ++round) {
                 if (!ok) e = ld_relaxed(Ein + min(c, nm1));
                 ok = !need || (uint32_t)(e >> 32) == tag_in;
-            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
+                if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
+                if ((round & 7) == 0) {
+                    if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
+                    if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                        raise_abort(A.abort_flag);
+                        return false;
+                    }
+                }
+            }
         }
         I = (uint32_t)e;
         return true;
@@ -389,17 +394,21 @@ __device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, 
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+            // poll as band_pass does: the ring words first, abort and clock every 8th round
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-            do {
-                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
-                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
-                    G->stop = 1u;
-                    raise_abort(A.abort_flag);
-                    return false;
-                }
+            for (int round = 1;; ++round) {
                 if (!ok) e = ld_relaxed(Ein + min(c, nm1));
                 ok = !need || (uint32_t)(e >> 32) == tag_in;
-            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
+                if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
+                if ((round & 7) == 0) {
+                    if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
+                    if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                        G->stop = 1u;
+                        raise_abort(A.abort_flag);
+                        return false;
+                    }
+                }
+            }
         }
         I = (uint32_t)e;
         return true;

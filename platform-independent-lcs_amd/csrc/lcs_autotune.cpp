@@ -21,10 +21,23 @@ extern "C" int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t 
     if (!ctx || m < 1 || n < 1 || alphabet < 1 || ncand < 1 || !candidates || !cost_ms || reps < 1) return LCS_EINVAL;
     if (mode != 0 && mode != 1) return LCS_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
-    // pairs: A uniform, B = A mutated (substitutions and indels 4:1)
-    const double cells = (double)m * (double)n;
-    // mode 1: about 2^28 cells per timed launch (at least 8 pairs, at most 4096)
-    const int K = mode == 0 ? 1 : (int)std::min(4096.0, std::max(8.0, 268435456.0 / cells));
+    // pairs: A uniform, B = A mutated (substitutions and indels 4:1).
+    // Mode 1 prices a pair inside a batch that keeps every SIMD busy, as a
+    // large batch of uneven pairs does: K pairs give even the coarsest
+    // schedule (64*32 rows per band) two bands per SIMD, and the columns are
+    // cut to n_s so that a timed launch stays near 2^30 cells; a band's time
+    // grows linearly in its columns, so the cost is scaled back by
+    // (n + 64) / (n_s + 64).
+    const int32_t n_full = n;
+    int K = 1;
+    if (mode == 1) {
+        const long long simds = 4LL * ctx->num_cus;
+        const long long coarse = std::max<long long>(1, bands_of(m, n, 32));
+        K = (int)std::min<long long>(4096, std::max<long long>(8, (2 * simds + coarse - 1) / coarse));
+        const long long cap = std::max<long long>(1024, (1LL << 30) / ((long long)K * m));
+        n = (int32_t)std::min<long long>(n, cap);
+    }
+    const double scale = (double)(n_full + 64) / (double)(n + 64);
     std::vector<int32_t> a_all((size_t)K * m), b_all((size_t)K * n);
     std::vector<int64_t> a_off((size_t)K), b_off((size_t)K);
     std::vector<int32_t> ms((size_t)K, m), ns((size_t)K, n);
@@ -71,7 +84,7 @@ extern "C" int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t 
             if (rc) return rc;
             double ms_k = 0;
             lcs_ctx_info(ctx, nullptr, &ms_k);
-            if (r) best = std::min(best, (float)(ms_k / K));
+            if (r) best = std::min(best, (float)(ms_k / K * scale));
         }
         cost_ms[c] = best;
     }

@@ -16,6 +16,8 @@ import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "liblcs_hip.so")
+# LCS_HIP_LIB names another build of the same library (A/B timing of kernel builds)
+LIB_PATH = os.environ.get("LCS_HIP_LIB", LIB_PATH)
 
 LCS_OK = 0
 ERRORS = {
