@@ -35,3 +35,13 @@ size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k);
 // staggered_multiply<row_glue>(p, q, k) on device pointers.
 int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
                          int32_t row_glue, int32_t *out, int32_t *ws, hipStream_t stream);
+// Several independent products; their cores share one launch.  ws of each
+// holds lcsb_multiply_ws_words(p_size, q_size, k) words.
+struct MulSpec {
+    const int32_t *p;
+    int32_t p_size;
+    const int32_t *q;
+    int32_t q_size, k, row_glue;
+    int32_t *out, *ws;
+};
+int lcsb_multiply_many_device(const MulSpec *specs, int count, hipStream_t stream);

@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "braid_kernels.h"
 
 namespace lcsb {
@@ -384,30 +386,38 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-// staggered_multiply on device pointers; ws must hold lcsb_multiply_ws_words(...)
-int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
-                         int32_t row_glue, int32_t *out, int32_t *ws, hipStream_t stream)
+// staggered_multiply for `count` independent products on device pointers.
+// Each product's slicing and assembly are a few small launches; the cores of
+// all products go to one launch (one workgroup each), so the products of a
+// merge-tree level run side by side.  specs[i].ws holds
+// lcsb_multiply_ws_words(p_size, q_size, k) words.
+int lcsb_multiply_many_device(const MulSpec *specs, int count, hipStream_t stream)
 {
-    const int nmax = p_size > q_size ? p_size : q_size;
-    int32_t *q_c2r = ws;                  // q_size
-    int32_t *map_row = q_c2r + q_size;    // k
-    int32_t *map_col = map_row + k;       // k
-    int32_t *Pred = map_col + k;          // k
-    int32_t *Qred = Pred + k;             // k
-    int32_t *core = Qred + k;             // k
-    int32_t *counts = core + k;           // nblocks
-    const int nblocks_max = (nmax + kScanTile - 1) / kScanTile + 1;
-    int32_t *core_ws = counts + nblocks_max;
-    if (k > 0) {
-        hipLaunchKernelGGL(invert_kernel, dim3((q_size + 255) / 256), dim3(256), 0, stream, q, q_size, q_c2r);
+    std::vector<BraidCoreDesc> cores;
+    cores.reserve((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        const MulSpec &M = specs[i];
+        const int32_t p_size = M.p_size, q_size = M.q_size, k = M.k;
+        const int nmax = p_size > q_size ? p_size : q_size;
+        int32_t *q_c2r = M.ws;                // q_size
+        int32_t *map_row = q_c2r + q_size;    // k
+        int32_t *map_col = map_row + k;       // k
+        int32_t *Pred = map_col + k;          // k
+        int32_t *Qred = Pred + k;             // k
+        int32_t *core = Qred + k;             // k
+        int32_t *counts = core + k;           // nblocks
+        const int nblocks_max = (nmax + kScanTile - 1) / kScanTile + 1;
+        int32_t *core_ws = counts + nblocks_max;
+        if (k == 0) continue;
+        hipLaunchKernelGGL(invert_kernel, dim3((q_size + 255) / 256), dim3(256), 0, stream, M.q, q_size, q_c2r);
         // vertical slice of p: rows whose column lies in the glued range
-        SliceSpec sp{p, p_size, row_glue ? 0 : p_size - k, row_glue ? k : p_size};
+        SliceSpec sp{M.p, p_size, M.row_glue ? 0 : p_size - k, M.row_glue ? k : p_size};
         int nb = (p_size + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts);
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
         hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts, map_row, Pred, 0, k);
         // horizontal slice of q: columns whose row lies in the glued range
-        SliceSpec sq{q_c2r, q_size, row_glue ? q_size - k : 0, row_glue ? q_size : k};
+        SliceSpec sq{q_c2r, q_size, M.row_glue ? q_size - k : 0, M.row_glue ? q_size : k};
         nb = (q_size + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts);
         hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
@@ -419,13 +429,29 @@ int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int
         h.k = k;
         h.L = lcsb_core_L(k);
         h.scratch = core_ws;
-        int rc = lcsb_launch_core(&h, 1, stream);
+        cores.push_back(h);
+    }
+    if (!cores.empty()) {
+        const int rc = lcsb_launch_core(cores.data(), (int)cores.size(), stream);
         if (rc) return rc;
     }
-    AssembleArgs A{p, q, out, map_row, map_col, core, p_size, q_size, k, row_glue};
-    const int n = nmax > k ? nmax : k;
-    hipLaunchKernelGGL(assemble_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A);
+    for (int i = 0; i < count; ++i) {
+        const MulSpec &M = specs[i];
+        const int nmax = M.p_size > M.q_size ? M.p_size : M.q_size;
+        int32_t *map_row = M.ws + M.q_size, *map_col = map_row + M.k, *core = map_col + 3 * (size_t)M.k;
+        AssembleArgs A{M.p, M.q, M.out, map_row, map_col, core, M.p_size, M.q_size, M.k, M.row_glue};
+        const int n = nmax > M.k ? nmax : M.k;
+        hipLaunchKernelGGL(assemble_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// staggered_multiply on device pointers; ws must hold lcsb_multiply_ws_words(...)
+int lcsb_multiply_device(const int32_t *p, int32_t p_size, const int32_t *q, int32_t q_size, int32_t k,
+                         int32_t row_glue, int32_t *out, int32_t *ws, hipStream_t stream)
+{
+    const MulSpec M{p, p_size, q, q_size, k, row_glue, out, ws};
+    return lcsb_multiply_many_device(&M, 1, stream);
 }
 
 size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k)

@@ -218,22 +218,19 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
-            // The band above is not that far yet: poll.  Each round re-reads
-            // the missing words first; the abort flag and the clock (one more
-            // round trip each) are looked at every 8th round only.
+            // The band above is not that far yet: poll.  Every round also
+            // reads the abort flag and the clock; re-reading the ring words
+            // back to back without them was measured slower.
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-            for (int round = 1;; ++round) {
+            do {
+                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
+                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                    raise_abort(A.abort_flag);
+                    return false;
+                }
                 if (!ok) e = ld_relaxed(Ein + min(c, nm1));
                 ok = !need || (uint32_t)(e >> 32) == tag_in;
-                if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
-                if ((round & 7) == 0) {
-                    if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
-                    if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
-                        raise_abort(A.abort_flag);
-                        return false;
-                    }
-                }
-            }
+            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
         }
         I = (uint32_t)e;
         return true;
@@ -394,21 +391,18 @@ __device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, 
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
-            // poll as band_pass does: the ring words first, abort and clock every 8th round
+            // poll as band_pass does
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
-            for (int round = 1;; ++round) {
+            do {
+                if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
+                if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
+                    G->stop = 1u;
+                    raise_abort(A.abort_flag);
+                    return false;
+                }
                 if (!ok) e = ld_relaxed(Ein + min(c, nm1));
                 ok = !need || (uint32_t)(e >> 32) == tag_in;
-                if (__builtin_amdgcn_ballot_w64(!ok) == 0) break;
-                if ((round & 7) == 0) {
-                    if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
-                    if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
-                        G->stop = 1u;
-                        raise_abort(A.abort_flag);
-                        return false;
-                    }
-                }
-            }
+            } while (__builtin_amdgcn_ballot_w64(!ok) != 0);
         }
         I = (uint32_t)e;
         return true;

@@ -220,13 +220,15 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
     const int32_t W = (n + strips - 1) / strips;
     const int S = (n + W - 1) / W;
     size_t perm_words = (size_t)S * m + n;
-    size_t ws_words = lcsb_multiply_ws_words(m + n, m + n, m);
-    const size_t words = (size_t)S * m + n + 2 * perm_words + ws_words + (size_t)(m + n) + 2 * (size_t)(S + 1);
+    // one workspace per product of a merge-tree level (at most S/2 of them)
+    const size_t ws_words = lcsb_multiply_ws_words(m + n, m + n, m);
+    const size_t nws = (size_t)(S / 2 > 0 ? S / 2 : 1);
+    const size_t words = (size_t)S * m + n + 2 * perm_words + nws * ws_words + (size_t)(m + n) + 2 * (size_t)(S + 1);
     int rc = ctx->strip_ws.reserve(ctx, words * 4);
     if (rc) return rc;
     int32_t *hs = (int32_t *)ctx->strip_ws.ptr, *vs = hs + (size_t)S * m;
     int32_t *permA = vs + n, *permB = permA + perm_words, *mws = permB + perm_words;
-    int32_t *c2r = mws + ws_words;
+    int32_t *c2r = mws + nws * ws_words;
     int64_t *d_off = (int64_t *)(((uintptr_t)(c2r + m + n) + 7) & ~(uintptr_t)7);
     std::vector<PairDesc> pairs((size_t)S);
     std::vector<int64_t> off((size_t)S), size((size_t)S);
@@ -248,7 +250,7 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
     // enqueue_pairs staged the descriptors into ctx->descs; reuse them for the permutations
     if ((rc = upload(ctx, d_off, off.data(), off.size() * sizeof(int64_t), stream))) return rc;
     if ((rc = lcsk_launch_from_strands((const PairDesc *)ctx->descs.ptr, S, m + W, permA, d_off, stream))) return rc;
-    // merge tree
+    // merge tree: the products of a level share one core launch
     std::vector<int32_t *> cur((size_t)S);
     std::vector<int64_t> cur_size = size;
     for (int s = 0; s < S; ++s) cur[(size_t)s] = permA + off[(size_t)s];
@@ -256,16 +258,17 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
     while (cur.size() > 1) {
         std::vector<int32_t *> nxt;
         std::vector<int64_t> nxt_size;
+        std::vector<MulSpec> specs;
         int32_t *dst = dst_base;
         for (size_t i = 0; i + 1 < cur.size(); i += 2) {
             const int64_t out_size = cur_size[i] + cur_size[i + 1] - m;
-            rc = lcsb_multiply_device(cur[i], (int32_t)cur_size[i], cur[i + 1], (int32_t)cur_size[i + 1], m, 0, dst,
-                                      mws, stream);
-            if (rc) return rc;
+            specs.push_back(MulSpec{cur[i], (int32_t)cur_size[i], cur[i + 1], (int32_t)cur_size[i + 1], m, 0, dst,
+                                    mws + specs.size() * ws_words});
             nxt.push_back(dst);
             nxt_size.push_back(out_size);
             dst += out_size;
         }
+        if ((rc = lcsb_multiply_many_device(specs.data(), (int)specs.size(), stream))) return rc;
         if (cur.size() % 2) {
             // carry the odd permutation over to the next buffer
             if (hipMemcpyAsync(dst, cur.back(), (size_t)cur_size.back() * 4, hipMemcpyDeviceToDevice, stream) !=
