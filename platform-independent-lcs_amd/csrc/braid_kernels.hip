@@ -39,6 +39,9 @@ constexpr int kThreads = 1024;
 constexpr int kChunkRows = 32;
 constexpr uint16_t kLoFlag = 0x8000u;
 constexpr uint16_t kIdx = 0x7fffu;
+// cores up to this size keep keys and products in LDS (braid_core_lds_kernel)
+constexpr int kLdsCoreK = 16384;
+constexpr size_t lds_core_bytes(int K) { return (size_t)4 * K * sizeof(uint16_t) + ((size_t)K / 64 + 1) * 4; }
 
 // Exclusive prefix of the 0/1 flags of keys[0..K) (bit `bit`), written to
 // S[0..K] (S[K] = total).  Each thread owns a contiguous run of elements.
@@ -85,8 +88,139 @@ __device__ void split_depth(const int32_t *key, int32_t *key_new, int32_t *par, 
     __syncthreads();
 }
 
+// One bottom-up depth (node size s) of the core: lift the children's
+// products into node coordinates (the reference's inverse_mapping, :191-202)
+// and run the chunked ant passage (:123-189) for every node of the depth.
+// U, CI are in LDS; R (children's products) and Rn (this depth's) are
+// either global int32 or LDS uint16 arrays; pp, pq map depth d+1 positions
+// to depth d positions; bT, bD hold K/32 chunk entries.
+template <typename UT, typename RT, typename RNT>
+__device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT *Rn, const int32_t *__restrict__ pp,
+                                                const int32_t *__restrict__ pq, int K, int s, int32_t *bT,
+                                                int32_t *bD)
+{
+    const int h = s >> 1;
+    // lift the children's products into node coordinates
+#pragma unroll 4
+    for (int x = threadIdx.x; x < K; x += kThreads) {
+        const int p_pos = pp[x];
+        const int cseg = x & ~(h - 1);
+        const int q_pos = pq[cseg + (int)R[x]];
+        const int v0 = p_pos & ~(s - 1);
+        const uint16_t lo = (x & h) ? 0 : kLoFlag;
+        const int lr = p_pos - v0, lc = q_pos - v0;
+        U[p_pos] = (uint16_t)lc | lo;
+        CI[v0 + lc] = (uint16_t)lr | lo;
+        Rn[p_pos] = lc;
+    }
+    __syncthreads();
+    // chunk height: 32 rows, or s/128 for big nodes, balancing the
+    // boundary scans (O(s) each) against the walks (O(rows) each)
+    const int rc = s < kChunkRows ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
+    const int nch = s / rc;            // chunks per node
+    const int total = K / rc;          // chunks at this depth
+    // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc).
+    // A node's first chunk (j = 0) enters at T = 0; its entry is computed in
+    // the walk, so only chunks j > 0 (nodes of 64 rows or more, at most K/32
+    // chunks) store entries in bT, bD.
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int ch = wave; nch > 1 && ch < total; ch += kThreads / 64) {
+        const int v0 = (ch / nch) * s, j = ch % nch;
+        if (j == 0) continue;
+        const int Dr = s - j * rc;     // rows below are done; the ant enters row Dr-1
+        int T = 0, delta = 0;
+        {
+            // T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
+            // z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr, Y = #{lo rows >= Dr}
+            const int per = (s + 63) / 64;
+            const int c_lo = min(lane * per, s), c_hi = min(c_lo + per, s);
+            int zc = 0, yc = 0;
+            for (int c = c_lo; c < c_hi; ++c) {
+                const uint16_t e = CI[v0 + c];
+                const int lo_pt = (e & kLoFlag) != 0, row = e & kIdx;
+                zc += lo_pt ? (row >= Dr) : (row < Dr);
+                yc += lo_pt && row >= Dr;
+            }
+            // wave-wide sums
+            int Y = yc;
+            for (int off = 32; off > 0; off >>= 1) Y += __shfl_xor(Y, off);
+            int incl = zc;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off);
+                if (lane >= off) incl += v;
+            }
+            const int excl = incl - zc;
+            if (Y > 0) {
+                // the lane whose run holds the Y-th z column finds it
+                const bool mine = excl < Y && incl >= Y;
+                int pos = -1;
+                if (mine) {
+                    int cnt = excl;
+                    for (int c = c_lo; c < c_hi; ++c) {
+                        const uint16_t e = CI[v0 + c];
+                        cnt += (e & kLoFlag) ? ((e & kIdx) >= Dr) : ((e & kIdx) < Dr);
+                        if (cnt == Y) { pos = c; break; }
+                    }
+                }
+                const unsigned long long who = __ballot(mine);
+                const int src = __ffsll((long long)who) - 1;
+                pos = __shfl(pos, src);
+                T = pos + 1;
+            } else {
+                T = 0;
+            }
+            delta = 0;  // delta(Dr, T) = Z(T) - Y = 0 by the choice of T
+        }
+        if (lane == 0) {
+            // enter row Dr-1: delta(Dr-1, T) = delta(Dr, T) - [row Dr-1 counted at T]
+            const uint16_t e = U[v0 + Dr - 1];
+            const int col = e & kIdx;
+            delta -= (e & kLoFlag) ? (col >= T) : (col < T);
+            bT[ch] = T;
+            bD[ch] = delta;
+        }
+    }
+    __syncthreads();
+    for (int ch = threadIdx.x; ch < total; ch += kThreads) {
+        const int v0 = (ch / nch) * s, j = ch % nch;
+        const int r_hi = s - j * rc, r_lo = r_hi - rc;
+        int dr = r_hi - 1, dc, delta;
+        if (j == 0) {
+            // T(s) = 0, delta(s, 0) = 0; entering row s-1 subtracts it when it is lo
+            dc = 0;
+            delta = (U[v0 + s - 1] & kLoFlag) ? -1 : 0;
+        } else {
+            dc = bT[ch];
+            delta = bD[ch];
+        }
+        while (dr >= r_lo) {
+            if (dc > 0) {
+                const uint16_t el = CI[v0 + dc - 1];
+                const int d_left = delta - ((el & kLoFlag) ? ((el & kIdx) >= dr) : ((el & kIdx) < dr));
+                const uint16_t ed = U[v0 + dr];
+                const int d_down = delta + ((ed & kLoFlag) ? ((ed & kIdx) >= dc) : ((ed & kIdx) < dc));
+                if (d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
+            }
+            // delta(dr, s) >= 0 always; the dc < s test only bounds the walk
+            if (delta < 0 && dc < s) {
+                const uint16_t ec = CI[v0 + dc];
+                delta += (ec & kLoFlag) ? ((ec & kIdx) >= dr) : ((ec & kIdx) < dr);
+                ++dc;
+            } else {
+                if (dr > 0) {
+                    const uint16_t er = U[v0 + dr - 1];
+                    delta -= (er & kLoFlag) ? ((er & kIdx) >= dc) : ((er & kIdx) < dc);
+                }
+                --dr;
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // The sticky product of the k-permutations P (row -> shared column) and Q
 // (shared row -> column) for every descriptor; one workgroup per product.
+// Keys and products in global memory: cores up to K = 32768.
 __global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
 {
     const BraidCoreDesc D = batch.d[blockIdx.x];
@@ -119,114 +253,106 @@ __global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
     }
     for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
     __syncthreads();
-
     for (int d = L - 1; d >= 0; --d) {
-        const int s = K >> d, h = s >> 1;
-        const int32_t *pp = parP + (size_t)d * K, *pq = parQ + (size_t)d * K;  // depth d+1 -> d
-        // lift the children's products into node coordinates
-        for (int x = threadIdx.x; x < K; x += kThreads) {
-            const int p_pos = pp[x];
-            const int cseg = x & ~(h - 1);
-            const int q_pos = pq[cseg + R[x]];
-            const int v0 = p_pos & ~(s - 1);
-            const uint16_t lo = (x & h) ? 0 : kLoFlag;
-            const int lr = p_pos - v0, lc = q_pos - v0;
-            U[p_pos] = (uint16_t)lc | lo;
-            CI[v0 + lc] = (uint16_t)lr | lo;
-            Rn[p_pos] = lc;
-        }
-        __syncthreads();
-        // chunk height: 32 rows, or s/128 for big nodes, balancing the
-        // boundary scans (O(s) each) against the walks (O(rows) each)
-        const int rc = s < kChunkRows ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
-        const int nch = s / rc;            // chunks per node
-        const int total = K / rc;          // chunks at this depth
-        // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc)
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        for (int ch = wave; ch < total; ch += kThreads / 64) {
-            const int v0 = (ch / nch) * s, j = ch % nch;
-            const int Dr = s - j * rc;     // rows below are done; the ant enters row Dr-1
-            int T = 0, delta = 0;
-            if (j > 0) {
-                // T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
-                // z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr, Y = #{lo rows >= Dr}
-                const int per = (s + 63) / 64;
-                const int c_lo = min(lane * per, s), c_hi = min(c_lo + per, s);
-                int zc = 0, yc = 0;
-                for (int c = c_lo; c < c_hi; ++c) {
-                    const uint16_t e = CI[v0 + c];
-                    const int lo_pt = (e & kLoFlag) != 0, row = e & kIdx;
-                    zc += lo_pt ? (row >= Dr) : (row < Dr);
-                    yc += lo_pt && row >= Dr;
-                }
-                // wave-wide sums
-                int Y = yc;
-                for (int off = 32; off > 0; off >>= 1) Y += __shfl_xor(Y, off);
-                int incl = zc;
-                for (int off = 1; off < 64; off <<= 1) {
-                    const int v = __shfl_up(incl, off);
-                    if (lane >= off) incl += v;
-                }
-                const int excl = incl - zc;
-                if (Y > 0) {
-                    // the lane whose run holds the Y-th z column finds it
-                    const bool mine = excl < Y && incl >= Y;
-                    int pos = -1;
-                    if (mine) {
-                        int cnt = excl;
-                        for (int c = c_lo; c < c_hi; ++c) {
-                            const uint16_t e = CI[v0 + c];
-                            cnt += (e & kLoFlag) ? ((e & kIdx) >= Dr) : ((e & kIdx) < Dr);
-                            if (cnt == Y) { pos = c; break; }
-                        }
-                    }
-                    const unsigned long long who = __ballot(mine);
-                    const int src = __ffsll((long long)who) - 1;
-                    pos = __shfl(pos, src);
-                    T = pos + 1;
-                } else {
-                    T = 0;
-                }
-                delta = 0;  // delta(Dr, T) = Z(T) - Y = 0 by the choice of T
-            }
-            if (lane == 0) {
-                // enter row Dr-1: delta(Dr-1, T) = delta(Dr, T) - [row Dr-1 counted at T]
-                const uint16_t e = U[v0 + Dr - 1];
-                const int col = e & kIdx;
-                delta -= (e & kLoFlag) ? (col >= T) : (col < T);
-                bT[ch] = T;
-                bD[ch] = delta;
-            }
-        }
-        __syncthreads();
-        for (int ch = threadIdx.x; ch < total; ch += kThreads) {
-            const int v0 = (ch / nch) * s, j = ch % nch;
-            const int r_hi = s - j * rc, r_lo = r_hi - rc;
-            int dr = r_hi - 1, dc = bT[ch], delta = bD[ch];
-            while (dr >= r_lo) {
-                if (dc > 0) {
-                    const uint16_t el = CI[v0 + dc - 1];
-                    const int d_left = delta - ((el & kLoFlag) ? ((el & kIdx) >= dr) : ((el & kIdx) < dr));
-                    const uint16_t ed = U[v0 + dr];
-                    const int d_down = delta + ((ed & kLoFlag) ? ((ed & kIdx) >= dc) : ((ed & kIdx) < dc));
-                    if (d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
-                }
-                // delta(dr, s) >= 0 always; the dc < s test only bounds the walk
-                if (delta < 0 && dc < s) {
-                    const uint16_t ec = CI[v0 + dc];
-                    delta += (ec & kLoFlag) ? ((ec & kIdx) >= dr) : ((ec & kIdx) < dr);
-                    ++dc;
-                } else {
-                    if (dr > 0) {
-                        const uint16_t er = U[v0 + dr - 1];
-                        delta -= (er & kLoFlag) ? ((er & kIdx) >= dc) : ((er & kIdx) < dc);
-                    }
-                    --dr;
-                }
-            }
-        }
-        __syncthreads();
+        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD);
         int32_t *tmp = R; R = Rn; Rn = tmp;
+    }
+    for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
+}
+
+// One depth of the top-down split with the keys in LDS: a stable partition
+// of every aligned segment of size s by bit `bit` (zeros first), the ranks
+// from wave ballots and one scan of the 64-key group counts.
+// par[pos] = old position (global).  K >= 64.
+__device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t *par, int K, int s, int bit,
+                                int32_t *gpre)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ngroups = K >> 6;
+    const unsigned long long lt = (1ull << lane) - 1;
+    for (int g = wave; g < ngroups; g += kThreads / 64) {
+        const unsigned long long m = __ballot((key[g * 64 + lane] >> bit) & 1);
+        if (lane == 0) gpre[g] = __popcll(m);
+    }
+    __syncthreads();
+    if (wave == 0) {  // exclusive scan of the group counts, gpre[ngroups] = total
+        int carry = 0;
+        for (int base = 0; base < ngroups; base += 64) {
+            const int c = base + lane < ngroups ? gpre[base + lane] : 0;
+            int incl = c;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off);
+                if (lane >= off) incl += v;
+            }
+            if (base + lane < ngroups) gpre[base + lane] = carry + incl - c;
+            carry += __shfl(incl, 63);
+        }
+        if (lane == 0) gpre[ngroups] = carry;
+    }
+    __syncthreads();
+    for (int g = wave; g < ngroups; g += kThreads / 64) {
+        const int x = g * 64 + lane;
+        const uint16_t kv = key[x];
+        const int b = (kv >> bit) & 1;
+        const unsigned long long m = __ballot(b);
+        const int seg = x & ~(s - 1);
+        int before, ones;
+        if (s >= 64) {
+            const int gs = seg >> 6;
+            before = gpre[g] - gpre[gs] + __popcll(m & lt);
+            ones = gpre[gs + (s >> 6)] - gpre[gs];
+        } else {
+            const unsigned long long segmask = ((1ull << s) - 1) << (lane & ~(s - 1));
+            before = __popcll(m & lt & segmask);
+            ones = __popcll(m & segmask);
+        }
+        const int pos = b ? seg + (s - ones) + before : seg + (x - seg - before);
+        key_new[pos] = kv;
+        par[pos] = x;
+    }
+    __syncthreads();
+}
+
+// The same product with keys, both products and the chunk entries in LDS:
+// cores up to K = kLdsCoreK.  LDS: 4K uint16 (keys / U, CI, R, Rn) plus
+// K/64+1 group counts.
+__global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batch)
+{
+    const BraidCoreDesc D = batch.d[blockIdx.x];
+    const int k = D.k, L = D.L, K = 1 << L;
+    extern __shared__ uint16_t lds[];
+    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    uint16_t *key = lds, *key2 = lds + K;             // top-down (P, then Q)
+    uint16_t *U = lds, *CI = lds + K;                 // bottom-up (reuses the key space)
+    uint16_t *R = lds + 2 * (size_t)K, *Rn = lds + 3 * (size_t)K;
+    int32_t *gpre = (int32_t *)(lds + 4 * (size_t)K);
+    int32_t *parP = D.scratch;                        // L * K
+    int32_t *parQ = parP + (size_t)L * K;             // L * K
+
+    // P ordered by row (key = shared column)
+    for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(x < k ? D.P[x] : x);
+    __syncthreads();
+    for (int d = 0; d < L; ++d) {
+        split_depth_lds(key, key2, parP + (size_t)d * K, K, K >> d, L - d - 1, gpre);
+        uint16_t *t = key; key = key2; key2 = t;
+    }
+    // Q ordered by column (key = shared row)
+    key = lds;
+    key2 = lds + K;
+    for (int x = threadIdx.x; x < K; x += kThreads)
+        if (x >= k) key[x] = (uint16_t)x;
+    for (int r = threadIdx.x; r < k; r += kThreads)
+        if ((uint32_t)D.Q[r] < (uint32_t)k) key[D.Q[r]] = (uint16_t)r;
+    __syncthreads();
+    for (int d = 0; d < L; ++d) {
+        split_depth_lds(key, key2, parQ + (size_t)d * K, K, K >> d, L - d - 1, gpre);
+        uint16_t *t = key; key = key2; key2 = t;
+    }
+    for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
+    __syncthreads();
+    for (int d = L - 1; d >= 0; --d) {
+        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD);
+        uint16_t *tmp = R; R = Rn; Rn = tmp;
     }
     for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
 }
@@ -348,15 +474,16 @@ using namespace lcsb;
 
 size_t lcsb_core_scratch_words(int32_t k)
 {
-    int L = 0;
-    while ((1 << L) < k) ++L;
+    const int L = lcsb_core_L(k);
     const size_t K = (size_t)1 << L;
     return (size_t)9 * K + 1 + (size_t)2 * L * K;
 }
 
+// Padded core size 2^L: at least 64 (the LDS kernel ranks keys by wave
+// ballots over groups of 64); padding adds fixed points only.
 int lcsb_core_L(int32_t k)
 {
-    int L = 0;
+    int L = 6;
     while ((1 << L) < k) ++L;
     return L;
 }
@@ -366,7 +493,9 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)braid_core_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(2 * kMaxCoreK * sizeof(uint16_t))) != hipSuccess)
+                                (int)(2 * kMaxCoreK * sizeof(uint16_t))) != hipSuccess ||
+            hipFuncSetAttribute((const void *)braid_core_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_core_bytes(kLdsCoreK)) != hipSuccess)
             return -5;
         attr_set = true;
     }
@@ -380,8 +509,11 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
             if ((1 << batch.d[i].L) > max_K) max_K = 1 << batch.d[i].L;
         }
         if (max_K > kMaxCoreK) return -34;
-        const size_t lds = (size_t)2 * max_K * sizeof(uint16_t);
-        hipLaunchKernelGGL(braid_core_kernel, dim3(cnt), dim3(kThreads), lds, stream, batch);
+        if (max_K <= kLdsCoreK)
+            hipLaunchKernelGGL(braid_core_lds_kernel, dim3(cnt), dim3(kThreads), lds_core_bytes(max_K), stream, batch);
+        else
+            hipLaunchKernelGGL(braid_core_kernel, dim3(cnt), dim3(kThreads), (size_t)2 * max_K * sizeof(uint16_t),
+                               stream, batch);
     }
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
