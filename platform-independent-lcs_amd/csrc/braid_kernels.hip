@@ -41,6 +41,9 @@ constexpr uint16_t kLoFlag = 0x8000u;
 constexpr uint16_t kIdx = 0x7fffu;
 // cores up to this size keep keys and products in LDS (braid_core_lds_kernel)
 constexpr int kLdsCoreK = 16384;
+// row chunks per node are at most 128 (chunk height max(32, s/128)); each wave
+// keeps 2 x kMaxChunks bucket counters in LDS for the boundary sweeps
+constexpr int kMaxChunks = 128;
 constexpr size_t lds_core_bytes(int K) { return (size_t)4 * K * sizeof(uint16_t) + ((size_t)K / 64 + 1) * 4; }
 
 // Exclusive prefix of the 0/1 flags of keys[0..K) (bit `bit`), written to
@@ -97,7 +100,7 @@ __device__ void split_depth(const int32_t *key, int32_t *key_new, int32_t *par, 
 template <typename UT, typename RT, typename RNT>
 __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT *Rn, const int32_t *__restrict__ pp,
                                                 const int32_t *__restrict__ pq, int K, int s, int32_t *bT,
-                                                int32_t *bD)
+                                                int32_t *bD, int *hist)
 {
     const int h = s >> 1;
     // lift the children's products into node coordinates
@@ -122,62 +125,109 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc).
     // A node's first chunk (j = 0) enters at T = 0; its entry is computed in
     // the walk, so only chunks j > 0 (nodes of 64 rows or more, at most K/32
-    // chunks) store entries in bT, bD.
+    // chunks) store entries in bT, bD.  For chunk j > 0 entering at row
+    // Dr = s - j rc:
+    //   T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
+    //   z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr,  Y = #{lo rows >= Dr},
+    // and delta(Dr, T) = Z(T) - Y = 0.  With the row bucket of a column,
+    // b(c) = (s-1-row(c)) / rc, z_j(c) = lo ? b(c) < j : b(c) >= j, so one
+    // wave finds all of a node's boundaries in two sweeps over its columns
+    // (64 per round, conflict-free): the histogram of the round's buckets
+    // (lo and hi, wave-local LDS counters) and a scan over the buckets give
+    // every boundary's count at once; lane L keeps boundaries 2L and 2L+1.
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int ch = wave; nch > 1 && ch < total; ch += kThreads / 64) {
-        const int v0 = (ch / nch) * s, j = ch % nch;
-        if (j == 0) continue;
-        const int Dr = s - j * rc;     // rows below are done; the ant enters row Dr-1
-        int T = 0, delta = 0;
-        {
-            // T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
-            // z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr, Y = #{lo rows >= Dr}
-            const int per = (s + 63) / 64;
-            const int c_lo = min(lane * per, s), c_hi = min(c_lo + per, s);
-            int zc = 0, yc = 0;
-            for (int c = c_lo; c < c_hi; ++c) {
-                const uint16_t e = CI[v0 + c];
-                const int lo_pt = (e & kLoFlag) != 0, row = e & kIdx;
-                zc += lo_pt ? (row >= Dr) : (row < Dr);
-                yc += lo_pt && row >= Dr;
+    const unsigned long long lt = (1ull << lane) - 1;
+    if (nch > 1) {
+        int *hlo = hist + wave * 2 * kMaxChunks, *hhi = hlo + kMaxChunks;   // this wave's counters
+        const int j0 = 2 * lane, j1 = 2 * lane + 1;
+        for (int node = wave; node < K / s; node += kThreads / 64) {
+            const int v0 = node * s;
+            // sweep 1: Y_j = #{lo columns with bucket < j}
+            hlo[j0] = 0;
+            hlo[j1] = 0;
+            // the wave's LDS accesses run in order; the fences keep the
+            // compiler from moving counter stores, adds and loads across
+            // each other
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            for (int c0 = 0; c0 < s; c0 += 64) {
+                const uint16_t e = CI[v0 + c0 + lane];
+                if (e & kLoFlag) atomicAdd(&hlo[(s - 1 - (e & kIdx)) / rc], 1);
             }
-            // wave-wide sums
-            int Y = yc;
-            for (int off = 32; off > 0; off >>= 1) Y += __shfl_xor(Y, off);
-            int incl = zc;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            int a0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            int a1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            int incl = a0 + a1;
             for (int off = 1; off < 64; off <<= 1) {
                 const int v = __shfl_up(incl, off);
                 if (lane >= off) incl += v;
             }
-            const int excl = incl - zc;
-            if (Y > 0) {
-                // the lane whose run holds the Y-th z column finds it
-                const bool mine = excl < Y && incl >= Y;
-                int pos = -1;
-                if (mine) {
-                    int cnt = excl;
-                    for (int c = c_lo; c < c_hi; ++c) {
-                        const uint16_t e = CI[v0 + c];
-                        cnt += (e & kLoFlag) ? ((e & kIdx) >= Dr) : ((e & kIdx) < Dr);
-                        if (cnt == Y) { pos = c; break; }
+            const int Y0 = incl - a0 - a1, Y1 = Y0 + a0;   // exclusive prefixes at j0, j1
+            // sweep 2: running Z_j per boundary until Z_j reaches Y_j
+            int Z0 = 0, Z1 = 0, T0 = 0, T1 = 0;
+            bool open0 = j0 >= 1 && j0 < nch && Y0 > 0, open1 = j1 < nch && Y1 > 0;
+            for (int c0 = 0; c0 < s && __ballot(open0 || open1); c0 += 64) {
+                hlo[j0] = 0;
+                hlo[j1] = 0;
+                hhi[j0] = 0;
+                hhi[j1] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const uint16_t e = CI[v0 + c0 + lane];
+                const bool lo = (e & kLoFlag) != 0;
+                const int bk = (s - 1 - (e & kIdx)) / rc;
+                atomicAdd(lo ? &hlo[bk] : &hhi[bk], 1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                const int l0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                const int l1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                const int g0 = __hip_atomic_load(&hhi[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                const int g1 = __hip_atomic_load(&hhi[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                int il = l0 + l1, ih = g0 + g1;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int vl = __shfl_up(il, off), vh = __shfl_up(ih, off);
+                    if (lane >= off) {
+                        il += vl;
+                        ih += vh;
                     }
                 }
-                const unsigned long long who = __ballot(mine);
-                const int src = __ffsll((long long)who) - 1;
-                pos = __shfl(pos, src);
-                T = pos + 1;
-            } else {
-                T = 0;
+                const int hi_total = __shfl(ih, 63);
+                const int lp0 = il - l0 - l1, lp1 = lp0 + l0;      // lo columns with bucket < j
+                const int hp0 = ih - g0 - g1, hp1 = hp0 + g0;      // hi columns with bucket < j
+                const int cnt0 = lp0 + (hi_total - hp0), cnt1 = lp1 + (hi_total - hp1);
+                const bool cross0 = open0 && Z0 + cnt0 >= Y0, cross1 = open1 && Z1 + cnt1 >= Y1;
+                if (open0 && !cross0) Z0 += cnt0;
+                if (open1 && !cross1) Z1 += cnt1;
+                // boundaries reaching Y in this round: find the exact column, one at a time
+                const unsigned long long todo = __ballot(cross0), todo1 = __ballot(cross1);
+                for (int which = 0; which < 2; ++which) {
+                    unsigned long long t = which ? todo1 : todo;
+                    while (t) {
+                        const int l = __ffsll((long long)t) - 1;
+                        t &= t - 1;
+                        const int jj = 2 * l + which;
+                        const int need = __shfl(which ? Y1 - Z1 : Y0 - Z0, l);   // 1-based rank in this round
+                        const bool z = lo ? bk < jj : bk >= jj;
+                        const unsigned long long zm = __ballot(z);
+                        const bool hit = z && __popcll(zm & lt) == need - 1;
+                        const int pos = __ffsll((long long)__ballot(hit));  // 1-based lane of the column
+                        if (lane == l) {
+                            if (which) T1 = c0 + pos; else T0 = c0 + pos;
+                        }
+                    }
+                }
+                if (cross0) open0 = false;
+                if (cross1) open1 = false;
             }
-            delta = 0;  // delta(Dr, T) = Z(T) - Y = 0 by the choice of T
-        }
-        if (lane == 0) {
-            // enter row Dr-1: delta(Dr-1, T) = delta(Dr, T) - [row Dr-1 counted at T]
-            const uint16_t e = U[v0 + Dr - 1];
-            const int col = e & kIdx;
-            delta -= (e & kLoFlag) ? (col >= T) : (col < T);
-            bT[ch] = T;
-            bD[ch] = delta;
+            // entries: enter row Dr-1, delta(Dr-1, T) = -[row Dr-1 counted at T]
+            for (int w2 = 0; w2 < 2; ++w2) {
+                const int j = w2 ? j1 : j0, T = w2 ? T1 : T0;
+                if (j >= 1 && j < nch) {
+                    const int Dr = s - j * rc;
+                    const uint16_t e = U[v0 + Dr - 1];
+                    const int col = e & kIdx;
+                    const int ch = node * nch + j;
+                    bT[ch] = T;
+                    bD[ch] = -((e & kLoFlag) ? (col >= T) : (col < T));
+                }
+            }
         }
     }
     __syncthreads();
@@ -229,6 +279,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
     uint16_t *U = lds;            // node-local row -> node-local col | lo flag
     uint16_t *CI = lds + K;       // node-local col -> node-local row | lo flag
     __shared__ int32_t sums[kThreads];
+    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
     int32_t *keyP = D.scratch, *keyQ = keyP + K, *keyP2 = keyQ + K, *keyQ2 = keyP2 + K;
     int32_t *S = keyQ2 + K;               // K+1
     int32_t *R = S + K + 1, *Rn = R + K;  // per depth products
@@ -254,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
     for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
     __syncthreads();
     for (int d = L - 1; d >= 0; --d) {
-        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD);
+        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, hist);
         int32_t *tmp = R; R = Rn; Rn = tmp;
     }
     for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
@@ -322,6 +373,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
     const int k = D.k, L = D.L, K = 1 << L;
     extern __shared__ uint16_t lds[];
     __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
     uint16_t *key = lds, *key2 = lds + K;             // top-down (P, then Q)
     uint16_t *U = lds, *CI = lds + K;                 // bottom-up (reuses the key space)
     uint16_t *R = lds + 2 * (size_t)K, *Rn = lds + 3 * (size_t)K;
@@ -351,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
     for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
     __syncthreads();
     for (int d = L - 1; d >= 0; --d) {
-        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD);
+        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, hist);
         uint16_t *tmp = R; R = Rn; Rn = tmp;
     }
     for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
