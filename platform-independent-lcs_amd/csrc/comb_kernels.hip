@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "cell_asm.h"
 #include "comb_kernels.h"
 
@@ -238,25 +240,37 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     uint32_t v = 0, O = 0, Osave = 0, I = 0;
     int32_t bq = 0, IB = pB;
     if (!take_inputs(0, I)) return false;
-    for (int k = 0; k < nchunks; ++k) {
+    // One chunk: prefetch chunk k+1, then publish chunk k-1 (the wait for the
+    // loads at the end then leaves the store in flight), comb C steps, take
+    // the next inputs.  The chunks run in three loops (ramp-in, full,
+    // ramp-out) so the hot loop holds only the unmasked steps and no
+    // register copies between masked and unmasked paths.
+    auto chunk = [&](auto colmask, int k) -> bool {
         const int t0 = k * C;
-        {   // prefetch chunk k+1, then publish chunk k-1: the wait for these
-            // loads at the end of the body then leaves the store in flight
+        {
             const int c = min(t0 + C + (lane & (C - 1)), nm1);
             pB = bsym[c];
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C);
-        if (t0 >= kWave - 1 && t0 + C <= n)
-            band_steps<TM, C, ROWMASK, false>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
-        else
-            band_steps<TM, C, ROWMASK, true>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
+        band_steps<TM, C, ROWMASK, decltype(colmask)::value>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
         Osave = O;
         if (k + 1 < nchunks) {
             IB = pB;
             if (!take_inputs(t0 + C, I)) return false;
         }
-    }
+        return true;
+    };
+    using Masked = std::integral_constant<bool, true>;
+    using Full = std::integral_constant<bool, false>;
+    auto full = [&](int k) { return k * C >= kWave - 1 && k * C + C <= n; };
+    int k = 0;
+    for (; k < nchunks && !full(k); ++k)
+        if (!chunk(Masked{}, k)) return false;
+    for (; k < nchunks && full(k); ++k)
+        if (!chunk(Full{}, k)) return false;
+    for (; k < nchunks; ++k)
+        if (!chunk(Masked{}, k)) return false;
     publish(Osave, (nchunks - 1) * C);
 #pragma unroll
     for (int k = 0; k < TM; ++k)
