@@ -123,6 +123,50 @@ int lcs_permutation_dominance(lcs_ctx *ctx, const int32_t *row_to_col, int32_t s
 int lcs_semilocal_string_substring(lcs_ctx *ctx, const int32_t *row_to_col, int32_t m, int32_t n, const int32_t *qi,
                                    const int32_t *qj, int32_t nq, int32_t *out);
 
+/* The four semi-local quadrants of P(a, b) (size m+n), query q = (q0, q1):
+ *   LCS_SEMI_STRING_SUBSTRING  LCS(a, b[q0:q1))        0 <= q0 <= q1 <= n
+ *   LCS_SEMI_SUBSTRING_STRING  LCS(a[q0:q1), b)        0 <= q0 <= q1 <= m
+ *   LCS_SEMI_PREFIX_SUFFIX     LCS(a[:q0), b[q1:))     0 <= q0 <= m, 0 <= q1 <= n
+ *   LCS_SEMI_SUFFIX_PREFIX     LCS(a[q0:), b[:q1))     0 <= q0 <= m, 0 <= q1 <= n
+ *   LCS_SEMI_DOMINANCE         #{ r >= q0 : row_to_col[r] < q1 }   (m, n: m+n = size)
+ * All are H(i, j) = (j - i) - #{ r >= m + i : P[r] < j } of the seaweed
+ * matrix, i in [-m, n], j in [0, m+n], read at fixed offsets. */
+#define LCS_SEMI_STRING_SUBSTRING 0
+#define LCS_SEMI_SUBSTRING_STRING 1
+#define LCS_SEMI_PREFIX_SUFFIX 2
+#define LCS_SEMI_SUFFIX_PREFIX 3
+#define LCS_SEMI_DOMINANCE 4
+
+/* One-shot quadrant queries from a host row_to_col (host pointers,
+ * synchronous).  Large query sets go through a dominance index built for
+ * the call, small ones through a direct count.  LCS_EINVAL for a query
+ * outside its quadrant's range.  No counterpart in the reference. */
+int lcs_semilocal_query(lcs_ctx *ctx, const int32_t *row_to_col, int32_t m, int32_t n, int32_t kind,
+                        const int32_t *q0, const int32_t *q1, int32_t nq, int32_t *out);
+
+/* A device-resident dominance index (wavelet matrix over row_to_col + 1):
+ * each query costs O(log size) dependent loads instead of a scan of the
+ * permutation.  row_to_col must hold values in [-1, size) (-1 = unset);
+ * anything else is LCS_EINVAL.  The index holds ceil(log2(size+1)) levels of
+ * 16 bytes per 64 rows, charged to the context's HBM budget; destroy it
+ * before the context.  No counterpart in the reference (it reads no scores
+ * from its permutations).
+ *   _build: host row_to_col; _build_device: device row_to_col (both
+ *     synchronous, the input may be freed on return).
+ *   _query: host q0/q1/out, synchronous, validated (LCS_EINVAL).
+ *   _query_device: device q0/q1/out, asynchronous on the context's stream;
+ *     a query outside its quadrant's range yields -1. */
+typedef struct lcs_dominance_index lcs_dominance_index;
+int lcs_dominance_index_build(lcs_ctx *ctx, const int32_t *row_to_col, int32_t size, lcs_dominance_index **out);
+int lcs_dominance_index_build_device(lcs_ctx *ctx, const int32_t *d_row_to_col, int32_t size,
+                                     lcs_dominance_index **out);
+int lcs_dominance_index_query(lcs_dominance_index *idx, int32_t m, int32_t n, int32_t kind, const int32_t *q0,
+                              const int32_t *q1, int32_t nq, int32_t *out);
+int lcs_dominance_index_query_device(lcs_dominance_index *idx, int32_t m, int32_t n, int32_t kind,
+                                     const int32_t *d_q0, const int32_t *d_q1, int32_t nq, int32_t *d_out);
+int64_t lcs_dominance_index_bytes(const lcs_dominance_index *idx);
+void lcs_dominance_index_destroy(lcs_dominance_index *idx);
+
 /* staggered_multiply<RowGlue>(p, q, k) (src/braid_multiplication.hpp:273-406)
  * on the GPU: the sticky-braid product of p (size p_size) and q (size
  * q_size) glued on k strands; row_glue = 0 glues p's last k columns to q's

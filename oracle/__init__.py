@@ -46,6 +46,7 @@ def _load():
     lib.orc_string_substring.argtypes = [_i32p, i32, i32, _i32p]
     lib.orc_llcs_dp.argtypes = [_i32p, i32, _i32p, i32]
     lib.orc_llcs_dp.restype = i32
+    lib.orc_semilocal.argtypes = [_i32p, i32, i32, i32, _i32p, _i32p, i32, _i32p]
     lib.orc_staggered_multiply.argtypes = [_i32p, i32, _i32p, i32, i32, i32, _i32p, _i32p]
     lib.orc_staggered_multiply.restype = i32
     lib.orc_sticky_bruteforce.argtypes = [_i32p, _i32p, i32, _i32p]
@@ -122,6 +123,14 @@ def string_substring(row_to_col, m: int, n: int) -> np.ndarray:
 def llcs_dp(a, b) -> int:
     a, b = _arr(a), _arr(b)
     return int(lib().orc_llcs_dp(_c(a), a.size, _c(b), b.size))
+
+
+def semilocal(row_to_col, m: int, n: int, kind: int, q0, q1) -> np.ndarray:
+    """Quadrant queries (kind 0..4, see lcs_oracle.h) by direct count."""
+    p, q0, q1 = _arr(row_to_col), _arr(q0), _arr(q1)
+    out = np.empty(max(q0.size, 1), np.int32)
+    lib().orc_semilocal(_c(p), m, n, kind, _c(q0), _c(q1), q0.size, _c(out))
+    return out[: q0.size].copy()
 
 
 def staggered_multiply(p_r2c, q_r2c, k: int, row_glue: bool):

@@ -29,6 +29,10 @@
  *       - orc_sticky_bruteforce (definition of the distance product) on
  *         random k <= 9 permutations.
  *   orc_llcs_dp    - textbook recurrence, checked on hand-computed cases.
+ *   orc_semilocal  - every query of all four quadrants equals the DP of the
+ *                    corresponding substrings (small random pairs, empty
+ *                    strings included); dominance counts equal numpy's count
+ *                    of the definition; out-of-range queries give -1.
  */
 #include "lcs_oracle.h"
 
@@ -212,6 +216,35 @@ void orc_string_substring(const int32_t *row_to_col, int32_t m, int32_t n, int32
         for (int32_t j = i; j <= n; ++j) out[i * W + j] = (j - i) - cnt[j];
     }
     free(cnt);
+}
+
+static int32_t dominance_count(const int32_t *p, int32_t size, int64_t r0, int64_t c0)
+{
+    int32_t cnt = 0;
+    for (int64_t r = r0 < 0 ? 0 : r0; r < size; ++r) cnt += (int64_t)p[r] < c0;
+    return cnt;
+}
+
+static int32_t seaweed_h(const int32_t *p, int32_t m, int32_t n, int64_t i, int64_t j)
+{
+    return (int32_t)(j - i) - dominance_count(p, m + n, m + i, j);
+}
+
+void orc_semilocal(const int32_t *row_to_col, int32_t m, int32_t n, int32_t kind,
+                   const int32_t *q0, const int32_t *q1, int32_t nq, int32_t *out)
+{
+    for (int32_t q = 0; q < nq; ++q) {
+        const int64_t x = q0[q], y = q1[q];
+        int32_t r = -1;
+        switch (kind) {
+        case 0: if (0 <= x && x <= y && y <= n) r = seaweed_h(row_to_col, m, n, x, y); break;
+        case 1: if (0 <= x && x <= y && y <= m) r = seaweed_h(row_to_col, m, n, -x, n + (m - y)) - (int32_t)x - (int32_t)(m - y); break;
+        case 2: if (0 <= x && x <= m && 0 <= y && y <= n) r = seaweed_h(row_to_col, m, n, y, n + (m - x)) - (int32_t)(m - x); break;
+        case 3: if (0 <= x && x <= m && 0 <= y && y <= n) r = seaweed_h(row_to_col, m, n, -x, y) - (int32_t)x; break;
+        case 4: r = dominance_count(row_to_col, m + n, x, y); break;
+        }
+        out[q] = r;
+    }
 }
 
 int32_t orc_llcs_dp(const int32_t *a, int32_t m, const int32_t *b, int32_t n)

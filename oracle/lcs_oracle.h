@@ -70,6 +70,19 @@ void orc_string_substring(const int32_t *row_to_col, int32_t m, int32_t n,
 /* Textbook O(mn) LCS length (brute force, independent of the comb). */
 int32_t orc_llcs_dp(const int32_t *a, int32_t m, const int32_t *b, int32_t n);
 
+/* Semi-local quadrant queries by direct count.  With the seaweed matrix
+ * H(i, j) = (j - i) - #{ r >= m + i : P[r] < j }, i in [-m, n], j in [0, m+n]
+ * (b padded with m wildcards on each side), kind:
+ *   0 LCS(a, b[q0:q1))    = H(q0, q1)
+ *   1 LCS(a[q0:q1), b)    = H(-q0, n + m - q1) - q0 - (m - q1)
+ *   2 LCS(a[:q0), b[q1:)) = H(q1, n + m - q0) - (m - q0)
+ *   3 LCS(a[q0:), b[:q1)) = H(-q0, q1) - q0
+ *   4 #{ r >= q0 : P[r] < q1 }
+ * -1 for a query outside its quadrant.  The reference reads no scores from
+ * its permutations; the read-out is the semi-local LCS definition. */
+void orc_semilocal(const int32_t *row_to_col, int32_t m, int32_t n, int32_t kind,
+                   const int32_t *q0, const int32_t *q1, int32_t nq, int32_t *out);
+
 /* staggered_multiply<RowGlue>(p, q, k) of braid_multiplication.hpp:273-406
  * with its steady ant (:5-270).  p, q are given by row_to_col; the product
  * has p_size + q_size - k points.  out_c2r may be NULL. */

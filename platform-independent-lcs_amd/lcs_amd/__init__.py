@@ -26,6 +26,7 @@ ERRORS = {
 }
 
 VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS = 0, 1, 2
+SEMI_STRING_SUBSTRING, SEMI_SUBSTRING_STRING, SEMI_PREFIX_SUFFIX, SEMI_SUFFIX_PREFIX, SEMI_DOMINANCE = 0, 1, 2, 3, 4
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -91,6 +92,15 @@ def lib() -> ctypes.CDLL:
                                        f32p]
     L.lcs_permutation_dominance.argtypes = [vp, _i32p, i32, _i32p, _i32p, i32, _i32p]
     L.lcs_semilocal_string_substring.argtypes = [vp, _i32p, i32, i32, _i32p, _i32p, i32, _i32p]
+    L.lcs_semilocal_query.argtypes = [vp, _i32p, i32, i32, i32, _i32p, _i32p, i32, _i32p]
+    L.lcs_dominance_index_build.argtypes = [vp, _i32p, i32, ctypes.POINTER(vp)]
+    L.lcs_dominance_index_build_device.argtypes = [vp, vp, i32, ctypes.POINTER(vp)]
+    L.lcs_dominance_index_query.argtypes = [vp, i32, i32, i32, _i32p, _i32p, i32, _i32p]
+    L.lcs_dominance_index_query_device.argtypes = [vp, i32, i32, i32, vp, vp, i32, vp]
+    L.lcs_dominance_index_bytes.argtypes = [vp]
+    L.lcs_dominance_index_bytes.restype = i64
+    L.lcs_dominance_index_destroy.argtypes = [vp]
+    L.lcs_dominance_index_destroy.restype = None
     L.lcs_server_create.argtypes = [vp, sz, i32, ctypes.POINTER(vp)]
     L.lcs_server_destroy.argtypes = [vp]
     L.lcs_server_submit.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, f32p, _i64p]
@@ -219,6 +229,14 @@ class Context:
             self._h, _p(p), p.size, _p(qr), _p(qc), qr.size, _p(out)))
         return out[: qr.size].copy()
 
+    def semilocal(self, row_to_col, m: int, n: int, kind: int, q0, q1) -> np.ndarray:
+        """Quadrant queries (SEMI_*) read from P(a, b) on the GPU."""
+        p, q0, q1 = _i32(row_to_col), _i32(q0), _i32(q1)
+        out = np.empty(max(q0.size, 1), np.int32)
+        _check("lcs_semilocal_query", lib().lcs_semilocal_query(
+            self._h, _p(p), m, n, kind, _p(q0), _p(q1), q0.size, _p(out)))
+        return out[: q0.size].copy()
+
     def batch_comb(self, a_list, b_list, scheds=None, use_graph: bool = True):
         """Comb every pair (a_list[p], b_list[p]) in one batched launch."""
         npairs = len(a_list)
@@ -242,6 +260,49 @@ class Context:
         _check("lcs_batch_comb", rc)
         return ([h[a_off[p]: a_off[p] + m[p]].copy() for p in range(npairs)],
                 [v[b_off[p]: b_off[p] + n[p]].copy() for p in range(npairs)])
+
+
+class DominanceIndex:
+    """A device-resident dominance index of one permutation
+    (lcs_dominance_index_*): quadrant queries in O(log size) each."""
+
+    def __init__(self, ctx: Context, row_to_col=None, size: int | None = None, device_ptr: int | None = None):
+        self._ctx, self._h = ctx, ctypes.c_void_p()
+        if device_ptr is not None:
+            _check("lcs_dominance_index_build_device", lib().lcs_dominance_index_build_device(
+                ctx._h, ctypes.c_void_p(device_ptr), int(size), ctypes.byref(self._h)))
+            self.size = int(size)
+        else:
+            p = _i32(row_to_col)
+            _check("lcs_dominance_index_build", lib().lcs_dominance_index_build(
+                ctx._h, _p(p), p.size, ctypes.byref(self._h)))
+            self.size = p.size
+
+    def query(self, m: int, n: int, kind: int, q0, q1) -> np.ndarray:
+        q0, q1 = _i32(q0), _i32(q1)
+        out = np.empty(max(q0.size, 1), np.int32)
+        _check("lcs_dominance_index_query", lib().lcs_dominance_index_query(
+            self._h, m, n, kind, _p(q0), _p(q1), q0.size, _p(out)))
+        return out[: q0.size].copy()
+
+    def query_device(self, m: int, n: int, kind: int, d_q0: int, d_q1: int, nq: int, d_out: int) -> None:
+        _check("lcs_dominance_index_query_device", lib().lcs_dominance_index_query_device(
+            self._h, m, n, kind, ctypes.c_void_p(d_q0), ctypes.c_void_p(d_q1), nq, ctypes.c_void_p(d_out)))
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().lcs_dominance_index_bytes(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().lcs_dominance_index_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchLayout:
@@ -483,7 +544,8 @@ def workload_single(name: str, seed: int = 1) -> Pair:
     if name == "uniform4096":
         return Pair(gen_uniform(seed, 4096, 4), gen_uniform(seed + 1, 4096, 4), 4)
     if name.startswith("mutated"):
-        size = int(eval(name[len("mutated"):].replace("^", "**")))
+        spec = name[len("mutated"):]
+        size = 1 << int(spec[2:]) if spec.startswith("2^") else int(spec)
         a = gen_uniform(seed, size, 4)
         b = gen_mutate(seed + 1, a, 4, 0.10, 0.02, size)
         return Pair(a, b, 4, 0.12)

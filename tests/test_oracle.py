@@ -74,6 +74,35 @@ def test_semilocal_windows_equal_dp():
                 assert H[i, j] == oracle.llcs_dp(a, b[i:j])
 
 
+def test_semilocal_quadrants_equal_dp():
+    """Every query of the four quadrants against the DP of its substrings,
+    empty strings included; dominance against the definition; -1 outside."""
+    for seed in range(30):
+        rng = np.random.default_rng(300 + seed)
+        m, n, s = int(rng.integers(0, 14)), int(rng.integers(0, 14)), int(rng.integers(1, 4))
+        a, b = rnd(seed + 7, m, n, s)
+        r2c = oracle.permutation(a, b)
+        L = oracle.llcs_dp
+        qs = {0: [(i, j) for i in range(n + 1) for j in range(i, n + 1)],
+              1: [(k, e) for k in range(m + 1) for e in range(k, m + 1)],
+              2: [(p, i) for p in range(m + 1) for i in range(n + 1)],
+              3: [(k, j) for k in range(m + 1) for j in range(n + 1)]}
+        want = {0: lambda x, y: L(a, b[x:y]), 1: lambda x, y: L(a[x:y], b),
+                2: lambda x, y: L(a[:x], b[y:]), 3: lambda x, y: L(a[x:], b[:y])}
+        for kind, q in qs.items():
+            q0, q1 = np.array(q, np.int32).T
+            got = oracle.semilocal(r2c, m, n, kind, q0, q1)
+            assert list(got) == [want[kind](x, y) for x, y in q], (seed, kind)
+        r0 = rng.integers(-2, m + n + 2, 40).astype(np.int32)
+        c0 = rng.integers(-2, m + n + 2, 40).astype(np.int32)
+        dom = oracle.semilocal(r2c, m, n, 4, r0, c0)
+        assert list(dom) == [int((r2c[max(int(x), 0):] < y).sum()) for x, y in zip(r0, c0)]
+    r2c = oracle.permutation([1, 2], [2, 1, 2])
+    bad = oracle.semilocal(r2c, 2, 3, 0, [2, -1, 0], [1, 3, 4])
+    assert list(bad) == [-1, -1, -1]
+    assert list(oracle.semilocal(r2c, 2, 3, 1, [0, 1], [3, 1])) == [-1, 0]
+
+
 def test_dp_hand_cases():
     assert oracle.llcs_dp([1, 2, 3, 4], [2, 4]) == 2
     assert oracle.llcs_dp([1, 2, 3], [3, 2, 1]) == 1
