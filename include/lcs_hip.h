@@ -177,6 +177,26 @@ int lcs_braid_multiply(lcs_ctx *ctx, const int32_t *p_row_to_col, int32_t p_size
                        const int32_t *q_row_to_col, int32_t q_size, int32_t k, int32_t row_glue,
                        int32_t *out_row_to_col);
 
+/* ---- score-only LCS length (bit-parallel) --------------------------------
+ * The length of the LCS of a and b alone, by the word-parallel LCS
+ * recurrence on the GPU (one bit per cell instead of one strand); the
+ * counterpart of the reference's disabled binary bit-parallel LLCS
+ * (Lcs_Prefix_Binary_Reference, src/lcs_reference.hpp:135-211), for any
+ * alphabet of at most 240 symbols.  Symbols of a and b must lie in
+ * [0, sigma), 1 <= sigma <= 240; a symbol outside gives LCS_ERANGE (host
+ * entry points) or a length of -1 (device entry point).  Equal to
+ * #{v_out[j] < m} of lcs_comb on the same pair.
+ *   lcs_llcs: host pointers, synchronous.
+ *   lcs_llcs_device: device pointers, asynchronous on the context's stream;
+ *     *d_llcs is a device int32.
+ *   lcs_llcs_batch: npairs pairs in one launch; a_all/b_all host arrays
+ *     with per-pair offsets and lengths (as lcs_batch_comb); out[p]. */
+int lcs_llcs(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n, int32_t sigma, int32_t *llcs);
+int lcs_llcs_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n, int32_t sigma,
+                    int32_t *d_llcs);
+int lcs_llcs_batch(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all, const int64_t *a_off, const int32_t *m,
+                   const int32_t *b_all, const int64_t *b_off, const int32_t *n, int32_t sigma, int32_t *out);
+
 /* ---- batched pairs ------------------------------------------------------
  * No counterpart in the reference (it combs one pair per call); the batch
  * form the reference's harness would call once per line of its run script

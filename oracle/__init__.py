@@ -46,6 +46,8 @@ def _load():
     lib.orc_string_substring.argtypes = [_i32p, i32, i32, _i32p]
     lib.orc_llcs_dp.argtypes = [_i32p, i32, _i32p, i32]
     lib.orc_llcs_dp.restype = i32
+    lib.orc_llcs_bitpar.argtypes = [_i32p, i32, _i32p, i32]
+    lib.orc_llcs_bitpar.restype = i32
     lib.orc_semilocal.argtypes = [_i32p, i32, i32, i32, _i32p, _i32p, i32, _i32p]
     lib.orc_staggered_multiply.argtypes = [_i32p, i32, _i32p, i32, i32, i32, _i32p, _i32p]
     lib.orc_staggered_multiply.restype = i32
@@ -123,6 +125,12 @@ def string_substring(row_to_col, m: int, n: int) -> np.ndarray:
 def llcs_dp(a, b) -> int:
     a, b = _arr(a), _arr(b)
     return int(lib().orc_llcs_dp(_c(a), a.size, _c(b), b.size))
+
+
+def llcs_bitpar(a, b) -> int:
+    """LCS length by the word-parallel recurrence (64-bit words, one thread)."""
+    a, b = _arr(a), _arr(b)
+    return int(lib().orc_llcs_bitpar(_c(a), a.size, _c(b), b.size))
 
 
 def semilocal(row_to_col, m: int, n: int, kind: int, q0, q1) -> np.ndarray:

@@ -29,6 +29,9 @@
  *       - orc_sticky_bruteforce (definition of the distance product) on
  *         random k <= 9 permutations.
  *   orc_llcs_dp    - textbook recurrence, checked on hand-computed cases.
+ *   orc_llcs_bitpar - equals orc_llcs_dp on random pairs (alphabets 1..30,
+ *                    negative and 32-bit symbols, word-boundary lengths
+ *                    63/64/65/127/128/129, empty strings).
  *   orc_semilocal  - every query of all four quadrants equals the DP of the
  *                    corresponding substrings (small random pairs, empty
  *                    strings included); dominance counts equal numpy's count
@@ -216,6 +219,54 @@ void orc_string_substring(const int32_t *row_to_col, int32_t m, int32_t n, int32
         for (int32_t j = i; j <= n; ++j) out[i * W + j] = (j - i) - cnt[j];
     }
     free(cnt);
+}
+
+static int cmp_i32(const void *x, const void *y)
+{
+    const int32_t a = *(const int32_t *)x, b = *(const int32_t *)y;
+    return (a > b) - (a < b);
+}
+
+/* lcs_reference.hpp:135-211 (the disabled binary bit-parallel LLCS) packs
+ * strands into words; this is the word-parallel recurrence for any alphabet. */
+int32_t orc_llcs_bitpar(const int32_t *a, int32_t m, const int32_t *b, int32_t n)
+{
+    if (m <= 0 || n <= 0) return 0;
+    const int64_t W = ((int64_t)m + 63) / 64;
+    int32_t *sym = (int32_t *)malloc(sizeof(int32_t) * (size_t)m);
+    memcpy(sym, a, sizeof(int32_t) * (size_t)m);
+    qsort(sym, (size_t)m, sizeof(int32_t), cmp_i32);
+    int64_t k = 0;
+    for (int64_t i = 0; i < m; ++i)
+        if (i == 0 || sym[i] != sym[i - 1]) sym[k++] = sym[i];
+    uint64_t *M = (uint64_t *)calloc((size_t)(k * W), sizeof(uint64_t));
+    for (int64_t i = 0; i < m; ++i) {
+        const int32_t *p = (const int32_t *)bsearch(&a[i], sym, (size_t)k, sizeof(int32_t), cmp_i32);
+        M[(p - sym) * W + i / 64] |= 1ull << (i % 64);
+    }
+    uint64_t *V = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)W);
+    for (int64_t w = 0; w < W; ++w) V[w] = ~0ull;
+    for (int32_t j = 0; j < n; ++j) {
+        const int32_t *p = (const int32_t *)bsearch(&b[j], sym, (size_t)k, sizeof(int32_t), cmp_i32);
+        if (!p) continue;   /* matches nothing: V unchanged */
+        const uint64_t *Mc = M + (p - sym) * W;
+        uint64_t carry = 0;
+        for (int64_t w = 0; w < W; ++w) {
+            const uint64_t v = V[w], u = v & Mc[w];
+            const uint64_t s1 = v + u, s = s1 + carry;
+            carry = (uint64_t)(s1 < v) | (uint64_t)(s < s1);
+            V[w] = s | (v & ~Mc[w]);
+        }
+    }
+    int64_t zeros = 0;
+    for (int64_t w = 0; w < W; ++w) {
+        uint64_t valid = (w == W - 1 && m % 64) ? ((1ull << (m % 64)) - 1) : ~0ull;
+        zeros += __builtin_popcountll(~V[w] & valid);
+    }
+    free(sym);
+    free(M);
+    free(V);
+    return (int32_t)zeros;
 }
 
 static int32_t dominance_count(const int32_t *p, int32_t size, int64_t r0, int64_t c0)

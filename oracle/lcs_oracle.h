@@ -70,6 +70,16 @@ void orc_string_substring(const int32_t *row_to_col, int32_t m, int32_t n,
 /* Textbook O(mn) LCS length (brute force, independent of the comb). */
 int32_t orc_llcs_dp(const int32_t *a, int32_t m, const int32_t *b, int32_t n);
 
+/* LCS length by the word-parallel recurrence over 64-bit words (the
+ * algorithm the reference's disabled Lcs_Prefix_Binary_Reference,
+ * lcs_reference.hpp:135-211, packs into words; Hyyro's form of the
+ * Allison-Dix recurrence): V = all ones over m bits; per symbol c of b,
+ * U = V & M[c], V = (V + U) | (V & ~M[c]); LCS = zero bits of V.  Any int32
+ * symbols (the masks are built per distinct symbol of a).  threads > 1
+ * splits nothing (the recurrence is sequential in b); it exists for the
+ * timing harness only and must be 1. */
+int32_t orc_llcs_bitpar(const int32_t *a, int32_t m, const int32_t *b, int32_t n);
+
 /* Semi-local quadrant queries by direct count.  With the seaweed matrix
  * H(i, j) = (j - i) - #{ r >= m + i : P[r] < j }, i in [-m, n], j in [0, m+n]
  * (b padded with m wildcards on each side), kind:

@@ -92,6 +92,9 @@ def lib() -> ctypes.CDLL:
                                        f32p]
     L.lcs_permutation_dominance.argtypes = [vp, _i32p, i32, _i32p, _i32p, i32, _i32p]
     L.lcs_semilocal_string_substring.argtypes = [vp, _i32p, i32, i32, _i32p, _i32p, i32, _i32p]
+    L.lcs_llcs.argtypes = [vp, _i32p, i32, _i32p, i32, i32, _i32p]
+    L.lcs_llcs_device.argtypes = [vp, vp, i32, vp, i32, i32, vp]
+    L.lcs_llcs_batch.argtypes = [vp, i32, _i32p, _i64p, _i32p, _i32p, _i64p, _i32p, i32, _i32p]
     L.lcs_semilocal_query.argtypes = [vp, _i32p, i32, i32, i32, _i32p, _i32p, i32, _i32p]
     L.lcs_dominance_index_build.argtypes = [vp, _i32p, i32, ctypes.POINTER(vp)]
     L.lcs_dominance_index_build_device.argtypes = [vp, vp, i32, ctypes.POINTER(vp)]
@@ -228,6 +231,37 @@ class Context:
         _check("lcs_permutation_dominance", lib().lcs_permutation_dominance(
             self._h, _p(p), p.size, _p(qr), _p(qc), qr.size, _p(out)))
         return out[: qr.size].copy()
+
+    def llcs(self, a, b, sigma: int) -> int:
+        """LCS length by the bit-parallel kernel (symbols in [0, sigma))."""
+        a, b = _i32(a), _i32(b)
+        out = ctypes.c_int32()
+        _check("lcs_llcs", lib().lcs_llcs(self._h, _p(a), a.size, _p(b), b.size, sigma, ctypes.byref(out)))
+        return out.value
+
+    def llcs_device(self, d_a: int, m: int, d_b: int, n: int, sigma: int, d_out: int) -> None:
+        _check("lcs_llcs_device", lib().lcs_llcs_device(self._h, ctypes.c_void_p(d_a), m, ctypes.c_void_p(d_b), n,
+                                                        sigma, ctypes.c_void_p(d_out)))
+
+    def llcs_batch(self, a_list, b_list, sigma: int) -> np.ndarray:
+        """LCS length of every pair in one launch."""
+        npairs = len(a_list)
+        m = np.array([len(x) for x in a_list], np.int32)
+        n = np.array([len(x) for x in b_list], np.int32)
+        a_off = np.zeros(npairs, np.int64)
+        b_off = np.zeros(npairs, np.int64)
+        if npairs:
+            a_off[1:] = np.cumsum(m[:-1], dtype=np.int64)
+            b_off[1:] = np.cumsum(n[:-1], dtype=np.int64)
+        a_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in a_list]) if npairs else [])
+        b_all = _i32(np.concatenate([np.asarray(x, np.int32) for x in b_list]) if npairs else [])
+        a_all = a_all if a_all.size else np.zeros(1, np.int32)
+        b_all = b_all if b_all.size else np.zeros(1, np.int32)
+        out = np.empty(max(npairs, 1), np.int32)
+        _check("lcs_llcs_batch", lib().lcs_llcs_batch(
+            self._h, npairs, _p(a_all), a_off.ctypes.data_as(_i64p), _p(m), _p(b_all),
+            b_off.ctypes.data_as(_i64p), _p(n), sigma, _p(out)))
+        return out[:npairs].copy()
 
     def semilocal(self, row_to_col, m: int, n: int, kind: int, q0, q1) -> np.ndarray:
         """Quadrant queries (SEMI_*) read from P(a, b) on the GPU."""

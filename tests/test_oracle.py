@@ -103,6 +103,23 @@ def test_semilocal_quadrants_equal_dp():
     assert list(oracle.semilocal(r2c, 2, 3, 1, [0, 1], [3, 1])) == [-1, 0]
 
 
+def test_llcs_bitpar_equals_dp():
+    rng = np.random.default_rng(41)
+    sizes = [0, 1, 2, 63, 64, 65, 127, 128, 129]
+    for m in sizes:
+        for n in (0, 1, 5, 64, 130):
+            a, b = rng.integers(0, 3, m), rng.integers(0, 3, n)
+            assert oracle.llcs_bitpar(a, b) == oracle.llcs_dp(a, b), (m, n)
+    for seed in range(40):
+        rng = np.random.default_rng(900 + seed)
+        m, n, s = int(rng.integers(1, 400)), int(rng.integers(1, 400)), int(rng.integers(1, 31))
+        lo = int(rng.integers(-2 ** 31, 2 ** 31 - 64))
+        a = rng.integers(lo, lo + s, m).astype(np.int32)
+        b = rng.integers(lo, lo + s + 2, n).astype(np.int32)
+        assert oracle.llcs_bitpar(a, b) == oracle.llcs_dp(a, b), seed
+    assert oracle.llcs_bitpar([1, 2, 3, 4], [2, 4]) == 2
+
+
 def test_dp_hand_cases():
     assert oracle.llcs_dp([1, 2, 3, 4], [2, 4]) == 2
     assert oracle.llcs_dp([1, 2, 3], [3, 2, 1]) == 1
