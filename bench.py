@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--check", action="store_true", help="cross-check two schedules bit-for-bit")
     ap.add_argument("--no-zipf", action="store_true", help="skip the m=2^14, n=2^22 Zipf leg")
     ap.add_argument("--no-semilocal", action="store_true", help="skip the semi-local query leg")
+    ap.add_argument("--no-llcs", action="store_true", help="skip the bit-parallel LCS-length leg")
     ap.add_argument("--batch-pairs", type=int, default=10000, help="pairs of the batched leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests of the served leg (0: skip it)")
     ap.add_argument("--serve-rate", type=float, default=150.0, help="Poisson arrival rate (requests/s)")
@@ -261,6 +262,29 @@ def bench_semilocal(ctx, r2c, m: int, n: int, llcs: int, nq: int = 1 << 22):
     return out
 
 
+def bench_llcs(ctx, a, b, llcs_comb: int, steps: int = 3):
+    """Score-only LCS length of the same m=n=2^20 pair by the bit-parallel
+    kernel (one bit per cell); must equal the comb's LCS read-out."""
+    da, db, dout = H.DeviceArray.from_numpy(a), H.DeviceArray.from_numpy(b), H.DeviceArray(1)
+    ctx.llcs_device(da.ptr.value, a.size, db.ptr.value, b.size, 4, dout.ptr.value)   # warm
+    H.synchronize()
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.llcs_device(da.ptr.value, a.size, db.ptr.value, b.size, 4, dout.ptr.value)
+        H.synchronize()
+        ks.append(ctx.info()[1])
+    wall = (time.perf_counter() - t0) / steps
+    got = int(dout.numpy()[0])
+    for d in (da, db, dout):
+        d.free()
+    cells = float(a.size) * float(b.size)
+    k = float(np.median(ks))
+    return {"workload": f"LCS length of the m=n=2^{int(np.log2(a.size))} pair, bit-parallel (lcs_llcs_device)",
+            "kernel_ms": round(k, 3), "gcups": round(cells / (k / 1e3) / 1e9, 1), "ms_per_call": round(wall * 1e3, 3),
+            "llcs": got, "equals_comb": got == llcs_comb}
+
+
 def bench_serve(ctx, nreq: int, rate: float):
     """Poisson arrivals of pair comparisons, lengths log-uniform in
     [1024, 262144], alphabet 4 or 20 (50/50), B from A by the same mutation
@@ -384,6 +408,8 @@ def main():
         out["cross_schedule_bit_exact"] = checked
     if not args.no_semilocal:
         out["semilocal"] = bench_semilocal(ctx, r2c, m, n, llcs)
+    if not args.no_llcs:
+        out["llcs_bitparallel"] = bench_llcs(ctx, a, b, llcs)
     if not args.no_zipf:
         out["zipf_planted"] = bench_zipf(ctx)
     if args.batch_pairs > 0:

@@ -73,7 +73,8 @@ int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len);
 int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms);
 
 /* Diagnostics (no counterpart in the reference): record the start and end
- * time (100 MHz device clock) of each band of the following chain launches.
+ * time (100 MHz device clock) of each band of the following chain launches
+ * (lcs_comb* and lcs_llcs*).
  * max_bands > 0 arms tracing, 0 disarms it; with host_out != NULL copies
  * 2*max_bands words of the last launch's trace out instead. */
 int lcs_debug_band_trace(lcs_ctx *ctx, int32_t max_bands, uint64_t *host_out);
@@ -182,8 +183,8 @@ int lcs_braid_multiply(lcs_ctx *ctx, const int32_t *p_row_to_col, int32_t p_size
  * recurrence on the GPU (one bit per cell instead of one strand); the
  * counterpart of the reference's disabled binary bit-parallel LLCS
  * (Lcs_Prefix_Binary_Reference, src/lcs_reference.hpp:135-211), for any
- * alphabet of at most 240 symbols.  Symbols of a and b must lie in
- * [0, sigma), 1 <= sigma <= 240; a symbol outside gives LCS_ERANGE (host
+ * alphabet of at most 200 symbols.  Symbols of a and b must lie in
+ * [0, sigma), 1 <= sigma <= 200; a symbol outside gives LCS_ERANGE (host
  * entry points) or a length of -1 (device entry point).  Equal to
  * #{v_out[j] < m} of lcs_comb on the same pair.
  *   lcs_llcs: host pointers, synchronous.

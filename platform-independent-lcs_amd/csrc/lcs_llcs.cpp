@@ -21,8 +21,8 @@ struct LlcsPair {
 };
 
 // Steps from the first band's start to the last band's end: a band lags the
-// one below by about kLag steps (63 of skew, two chunks of hand-off, memory
-// latency), and bands beyond the resident waves run in later rounds.
+// one below by about kLag steps (63 of skew, two chunks of hand-off, the
+// fetch round trip), and bands beyond the resident ones run in later rounds.
 constexpr double kLag = 160.0;
 
 double pipeline_steps(int64_t rows, int64_t cols, int64_t resident)
@@ -32,7 +32,8 @@ double pipeline_steps(int64_t rows, int64_t cols, int64_t resident)
     return (double)rounds * (double)(cols + lcsl::kWave - 1) + (double)bands * kLag;
 }
 
-int resident_waves(const lcs_ctx *ctx) { return 8 * ctx->num_cus; }   // two per SIMD
+// bands in flight: a band is a workgroup of two waves (comb + fetch), four per CU
+int resident_bands(const lcs_ctx *ctx) { return 4 * ctx->num_cus; }
 
 // Enqueues one launch over `pairs` on `s`: the longer-pipelined orientation
 // is chosen per pair (LCS is symmetric).  Rings and descriptors live in the
@@ -41,7 +42,7 @@ int enqueue_llcs(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma
 {
     const int np = (int)pairs.size();
     if (np == 0) return LCS_OK;
-    const int64_t resident = resident_waves(ctx);
+    const int64_t resident = resident_bands(ctx);
     std::vector<lcsl::LlcsDesc> descs((size_t)np);
     size_t ring_words = 0;
     int max_ring = 0;
@@ -87,6 +88,8 @@ int enqueue_llcs(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma
     A.ticket = ctx->ctrl;
     A.abort_flag = ctx->ctrl + 1;
     A.timeout_ticks = ctx->timeout_ticks;
+    A.trace = ctx->band_trace;
+    A.trace_cap = ctx->band_trace_cap;
     const int blocks = (int)std::min<int64_t>((int64_t)tasks.size(), resident);
     return lcsl_launch(A, A.descs, np, max_ring, blocks, s) == 0 ? LCS_OK : LCS_EIO;
 }

@@ -9,7 +9,7 @@ constexpr int kWave = 64;
 constexpr int kWordRows = 32;                    // rows of the row string per lane
 constexpr int kBandRows = kWave * kWordRows;     // 2048 rows per band (one wave)
 constexpr int kChunk = 32;                       // steps per chunk = bits per ring word
-constexpr int kMaxSigma = 240;              // table + staging stay under 64 KiB of LDS
+constexpr int kMaxSigma = 200;              // table, staging and rings stay under 64 KiB of LDS
 
 // One LCS-length problem: `rows` (nrows symbols) is cut into bands of 2048
 // rows, one 32-bit word per lane; `cols` (ncols symbols) streams through the
@@ -25,7 +25,9 @@ struct LlcsDesc {
 };
 
 inline int32_t llcs_nbands(int32_t nrows) { return (int32_t)(((int64_t)nrows + kBandRows - 1) / kBandRows); }
-inline int32_t llcs_nchunks(int32_t ncols) { return (int32_t)(((int64_t)ncols + kWave - 1 + kChunk - 1) / kChunk); }
+// Chunks a band runs: lane 63 takes the last symbol at step ncols+62 and its
+// carry-out is collected one step later, in word (ncols-1)/32 + 2.
+inline int32_t llcs_nchunks(int32_t ncols) { return ncols > 0 ? (ncols - 1) / kChunk + 3 : 0; }
 
 struct LlcsArgs {
     const LlcsDesc *descs;
@@ -35,10 +37,19 @@ struct LlcsArgs {
     unsigned *ticket;            // zeroed by the prep kernel
     unsigned *abort_flag;        // zeroed by the prep kernel
     unsigned long long timeout_ticks;
+    unsigned long long *trace;   // nullable: [2t] start, [2t+1] end of task t (realtime ticks)
+    int32_t trace_cap;
+    int32_t pad;
 };
 
-// LDS: 96 staged addresses, one all-zero row (padding symbols), sigma rows
-inline size_t llcs_lds_bytes(int sigma) { return (size_t)(96 + kWave + sigma * kWave) * 4; }
+constexpr int kLdsSymRing = 2048, kLdsInRing = 128, kLdsOutRing = 64;
+// LDS: two buffers of 96 staged addresses, one all-zero row (padding
+// symbols), sigma rows, the symbol / carry-in / carry-out rings, 16 control
+// words
+inline size_t llcs_lds_bytes(int sigma)
+{
+    return (size_t)(2 * 96 + kWave + sigma * kWave + kLdsSymRing + kLdsInRing + kLdsOutRing + 16) * 4;
+}
 
 }  // namespace lcsl
 

@@ -28,9 +28,9 @@
 //  * Band hand-off: lane 63's carry-out bits, 32 per chunk, go to the band
 //    above as one tagged 64-bit word (tag << 32 | bits) stored with an
 //    agent-scope relaxed atomic; the band above polls for its tag, bounded in
-//    time, with the abort flag of the comb kernels.  Consumer chunk q needs
-//    words q+1 and q+2 of its producer (the skew of 63 steps is not a
-//    multiple of 32).
+//    time, with the abort flag of the comb kernels.  A carry-out bit is
+//    collected one step late, which aligns producer word c with symbols
+//    32(c-2) .. 32(c-2)+31: consumer chunk q reads word q+2.
 //  * Bands are handed out by an atomic ticket in order, so every wait is on a
 //    band that a resident wave already holds.
 #include <hip/hip_runtime.h>
@@ -43,12 +43,29 @@ namespace lcsl {
 #define LCSL_GLOBAL __attribute__((address_space(1)))
 typedef LCSL_GLOBAL unsigned long long gword;
 typedef LCSL_GLOBAL const unsigned long long gcword;
+typedef LCSL_GLOBAL const int32_t gsym;     // global, not flat: a flat load also counts against lgkmcnt,
+                                            // so every LDS wait would wait for it too
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) volatile uint32_t lds_vu32;
 
 constexpr int kStage = kChunk + kWave - 1;   // 95 symbols staged per chunk
-constexpr int kStageWords = 96;
+constexpr int kStageWords = 96;              // one staging buffer (two are used)
 constexpr int kZeroRowWords = kWave;         // row -1 of the table, all zero
 constexpr unsigned kNoTag = 0xffffffffu;
+constexpr int kSymRing = kLdsSymRing;        // LDS rings (powers of two): column symbols,
+constexpr int kInRing = kLdsInRing;          // carry words of the band below,
+constexpr int kOutRing = kLdsOutRing;        // this band's carry words
+constexpr int kSymBatch = 1024;              // symbols per fetch round
+// LDS control words of a workgroup
+constexpr int kTask = 0, kGo = 1, kStop = 2, kInAvail = 3, kInUsed = 4, kSymAvail = 5, kSymUsed = 6, kOutAvail = 7,
+              kOutUsed = 8;
+
+// The workgroup's LDS: staging buffers, the zero row and the match-mask
+// table (comb wave), the three rings and the control words (shared).
+struct Lds {
+    lds_u32 *stage, *tab, *sym, *in, *out;
+    lds_vu32 *ctl;
+};
 
 __device__ __forceinline__ unsigned long long ld_relaxed(gcword *p)
 {
@@ -63,48 +80,87 @@ __device__ __forceinline__ unsigned ld_flag(const unsigned *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Every lane loads the same ring word; readfirstlane makes the tag test and
-// the value provably wave-uniform, so every branch on them is a scalar branch
-// and the lanes of a wave never part (a per-lane exit would let the loop
-// around a band run with part of the wave).
-__device__ __forceinline__ uint32_t tag_of(unsigned long long e) { return __builtin_amdgcn_readfirstlane((uint32_t)(e >> 32)); }
-__device__ __forceinline__ uint32_t bits_of(unsigned long long e) { return __builtin_amdgcn_readfirstlane((uint32_t)e); }
-
-// Waits until ring word `p` carries `tag` (bounded in time).  Returns the
-// word's 32 bits, or raises the abort flag and clears `ok`.
-__device__ __forceinline__ uint32_t wait_word(const LlcsArgs &A, gcword *p, unsigned long long e, uint32_t tag,
-                                              bool &ok)
+// Waits (bounded in time) until control word `w` of the workgroup exceeds
+// `need`; `cache` keeps the last value seen.  False when the band must stop
+// (the other wave gave up, or the time ran out).
+__device__ __forceinline__ bool wait_ctl(const LlcsArgs &A, lds_vu32 *ctl, int w, uint32_t need, uint32_t &cache)
 {
-    if (tag_of(e) == tag) return bits_of(e);
+    if (cache > need) return true;
     const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     for (;;) {
-        if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) {
-            ok = false;
-            return 0;
-        }
+        cache = __builtin_amdgcn_readfirstlane(ctl[w]);
+        if (cache > need) return true;
+        if (__builtin_amdgcn_readfirstlane(ctl[kStop])) return false;
         if (__builtin_amdgcn_s_memrealtime() - t0 > A.timeout_ticks) {
             __hip_atomic_store(A.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = false;
-            return 0;
+            return false;
         }
-        __builtin_amdgcn_s_sleep(1);
-        e = ld_relaxed(p);
-        if (tag_of(e) == tag) return bits_of(e);
+        __builtin_amdgcn_s_sleep(0);
     }
 }
 
-// One step of one lane's 32 rows: U = V & M; (s, carry-out mask) = V + U +
-// carry-in mask; V = s | (V & ~M)  (v_bitop3 table 0xf4 over (s, V, M)).
-// Three VALU issues; the masks are SGPR pairs (bit l = lane l).
-__device__ __forceinline__ void step(uint32_t &V, uint32_t M, unsigned long long cin, unsigned long long &cout)
+// Sixteen steps of one lane's 32 rows.  Per step: the band's carry-in bit
+// for lane 0 (bit B of w) goes to SCC; VCC, the previous step's carry-out
+// mask, doubles into the carry-in mask (lane l-1's carry enters lane l) with
+// SCC as bit 0, and its old bit 63 (lane 63's carry-out of the previous
+// step) doubles into `o`; then U = V & M, (s, VCC) = V + U + VCC and
+// V = s | (V & ~M) (v_bitop3 table 0xf4 over (s, V, M)).  Four SALU and
+// three VALU issues.  VCC is written by VALU and read by SALU and the
+// reverse, which needs no wait states on gfx950; the carry state enters and
+// leaves in `c` so that no compiled code between blocks can disturb VCC.
+#define LCSL_STEP(B, K)                                   \
+    "s_bitcmp1_b32 %[w], " #B "\n\t"                    \
+    "s_addc_u32 vcc_lo, vcc_lo, vcc_lo\n\t"              \
+    "s_addc_u32 vcc_hi, vcc_hi, vcc_hi\n\t"              \
+    "s_addc_u32 %[o], %[o], %[o]\n\t"                    \
+    "v_and_b32 %[u], %[v], %[m" #K "]\n\t"               \
+    "v_addc_co_u32_e32 %[s], vcc, %[v], %[u], vcc\n\t"   \
+    "v_bitop3_b32 %[v], %[s], %[v], %[m" #K "] bitop3:0xf4\n\t"
+
+#define LCSL_OPERANDS(M, OFF)                                                                          \
+    : [v] "+v"(V), [o] "+s"(wout), [c] "+s"(carry), [u] "=&v"(u), [s] "=&v"(sum)                       \
+    : [w] "s"(win), [m0] "v"(M[OFF + 0]), [m1] "v"(M[OFF + 1]), [m2] "v"(M[OFF + 2]),                  \
+      [m3] "v"(M[OFF + 3]), [m4] "v"(M[OFF + 4]), [m5] "v"(M[OFF + 5]), [m6] "v"(M[OFF + 6]),           \
+      [m7] "v"(M[OFF + 7]), [m8] "v"(M[OFF + 8]), [m9] "v"(M[OFF + 9]), [m10] "v"(M[OFF + 10]),         \
+      [m11] "v"(M[OFF + 11]), [m12] "v"(M[OFF + 12]), [m13] "v"(M[OFF + 13]), [m14] "v"(M[OFF + 14]),   \
+      [m15] "v"(M[OFF + 15])                                                                            \
+    : "vcc", "scc"
+
+// The 32 steps of a chunk: carry-in bits come from w (bit 31 - t at step t),
+// carry-out bits of steps t-1 are collected into wout (the previous chunk's
+// last one first).
+// (readfirstlane returns int: each half goes through uint32_t so the low
+// half is not sign-extended into the high one)
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
 {
-    uint32_t u, s, vn;
-    asm("v_and_b32 %[u], %[v], %[m]\n\t"
-        "v_addc_co_u32_e64 %[s], %[co], %[v], %[u], %[ci]\n\t"
-        "v_bitop3_b32 %[vn], %[s], %[v], %[m] bitop3:0xf4"
-        : [u] "=&v"(u), [s] "=&v"(s), [vn] "=v"(vn), [co] "=&s"(cout)
-        : [v] "v"(V), [m] "v"(M), [ci] "s"(cin));
-    V = vn;
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ void chunk_steps(uint32_t &V, const uint32_t (&M)[kChunk], uint32_t win, uint32_t &wout,
+                                            unsigned long long &carry)
+{
+    // all three are wave-uniform by construction; readfirstlane lets the
+    // compiler see it across the early exits of the chunk loop
+    win = __builtin_amdgcn_readfirstlane(win);
+    wout = __builtin_amdgcn_readfirstlane(wout);
+    carry = uniform64(carry);
+    uint32_t u, sum;
+    asm("s_mov_b64 vcc, %[c]\n\t"
+        LCSL_STEP(31, 0) LCSL_STEP(30, 1) LCSL_STEP(29, 2) LCSL_STEP(28, 3)
+        LCSL_STEP(27, 4) LCSL_STEP(26, 5) LCSL_STEP(25, 6) LCSL_STEP(24, 7)
+        LCSL_STEP(23, 8) LCSL_STEP(22, 9) LCSL_STEP(21, 10) LCSL_STEP(20, 11)
+        LCSL_STEP(19, 12) LCSL_STEP(18, 13) LCSL_STEP(17, 14) LCSL_STEP(16, 15)
+        "s_mov_b64 %[c], vcc"
+        LCSL_OPERANDS(M, 0));
+    asm("s_mov_b64 vcc, %[c]\n\t"
+        LCSL_STEP(15, 0) LCSL_STEP(14, 1) LCSL_STEP(13, 2) LCSL_STEP(12, 3)
+        LCSL_STEP(11, 4) LCSL_STEP(10, 5) LCSL_STEP(9, 6) LCSL_STEP(8, 7)
+        LCSL_STEP(7, 8) LCSL_STEP(6, 9) LCSL_STEP(5, 10) LCSL_STEP(4, 11)
+        LCSL_STEP(3, 12) LCSL_STEP(2, 13) LCSL_STEP(1, 14) LCSL_STEP(0, 15)
+        "s_mov_b64 %[c], vcc"
+        LCSL_OPERANDS(M, 16));
 }
 
 // Column symbol j as a table row: [0, sigma) as is, -1 (the zero row) for
@@ -119,10 +175,18 @@ __device__ __forceinline__ int32_t code_of(int32_t s, bool inside, int sigma, bo
     return s;
 }
 
-__device__ bool band_llcs(const LlcsArgs &A, const LlcsDesc &D, int g, int lane, lds_u32 *stage, lds_u32 *tab)
+// Comb wave of a band: LDS only in its chunk loop.  Column symbols come from
+// the LDS symbol ring, the band below's carry words from the LDS in-ring, and
+// its own carry words go to the LDS out-ring; the fetch wave moves all three
+// between LDS and global memory.  (On gfx950 one counter, vmcnt, covers
+// loads and stores, so a global store in this loop would make the next
+// load's wait also wait for the store's device-scope write-through.)
+__device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int lane, const Lds &L)
 {
     const int sigma = A.sigma;
+    lds_u32 *tab = L.tab;
     const uint32_t tab_addr = (uint32_t)(uintptr_t)tab;
+    lds_vu32 *ctl = L.ctl;
 
     // match masks of this lane's 32 rows (each lane owns column `lane`)
     for (int c = -1; c < sigma; ++c) tab[c * kWave + lane] = 0u;
@@ -131,7 +195,7 @@ __device__ bool band_llcs(const LlcsArgs &A, const LlcsDesc &D, int g, int lane,
     for (int i = 0; i < kWordRows; ++i) {
         const int64_t r = r0 + i;
         if (r < D.nrows) {
-            const int32_t s = D.rows[r];
+            const int32_t s = ((gsym *)D.rows)[r];
             if ((uint32_t)s < (uint32_t)sigma)
                 tab[s * kWave + lane] |= 1u << i;
             else
@@ -140,72 +204,84 @@ __device__ bool band_llcs(const LlcsArgs &A, const LlcsDesc &D, int g, int lane,
     }
 
     const int32_t ncols = D.ncols, nchunks = D.nchunks;
-    gcword *rin = (gcword *)(D.ring + (size_t)(g & 1) * nchunks);
-    gword *rout = (gword *)(D.ring + (size_t)((g + 1) & 1) * nchunks);
-    const uint32_t tag_in = (uint32_t)g;
-    const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
     const bool has_in = g > 0, has_out = g + 1 < D.nbands;
-
-    // column symbols of chunk 0 (positions p = lane, lane + 64 of 95)
-    auto load_sym = [&](int64_t j) -> int32_t { return (j >= 0 && j < ncols) ? D.cols[j] : 0; };
-    int32_t s0 = load_sym((int64_t)lane - (kWave - 1));
-    int32_t s1 = lane < kStage - kWave ? load_sym((int64_t)lane + 1) : 0;
-
-    // carry words of the band below: chunk q needs words q+1 and q+2
+    uint32_t in_avail = 0, sym_avail = 0, out_used = 0;
     bool ok = true;
-    uint32_t w_hi = 0;
-    unsigned long long e_next = 0;
-    if (has_in) {
-        if (nchunks > 1) w_hi = wait_word(A, rin + 1, ld_relaxed(rin + 1), tag_in, ok);
-        if (!ok) return false;
-        if (nchunks > 2) e_next = ld_relaxed(rin + 2);
-    }
 
-    uint32_t V = 0xffffffffu;
-    unsigned long long cout = 0;
-    for (int q = 0; q < nchunks; ++q) {
-        // stage this chunk's symbols as table addresses
-        {
-            const int64_t j0 = (int64_t)q * kChunk - (kWave - 1);
-            const int64_t ja = j0 + lane, jb = j0 + kWave + lane;
-            // table address of row c, lane l, minus 4t (t = p + l - 63): read
-            // back with offset 4t by lane l at step t (p = t - l + 63)
-            const int32_t ca = code_of(s0, ja >= 0 && ja < ncols, sigma, bad);
-            stage[lane] = (uint32_t)((int32_t)tab_addr + ca * (kWave * 4) + (kWave - 1) * 4 - 4 * lane);
-            if (lane < kStage - kWave) {
-                const int32_t cb = code_of(s1, jb >= 0 && jb < ncols, sigma, bad);
-                stage[kWave + lane] = (uint32_t)((int32_t)tab_addr + cb * (kWave * 4) + (kWave - 1) * 4 - 4 * (kWave + lane));
-            }
-            s0 = load_sym(ja + kChunk);
-            s1 = lane < kStage - kWave ? load_sym(jb + kChunk) : 0;
+    // Chunk c covers steps [32c, 32c+32); lane l takes column symbol k - l at
+    // step k, so the chunk needs symbols 32c-63 .. 32c+31: staged position
+    // p = lane (symbol 32c-63+lane) and p = 64+lane (symbol 32c+1+lane).
+    auto raw_syms = [&](int c, int32_t &sa, int32_t &sb) -> bool {
+        const int64_t hi = min((int64_t)c * kChunk + kChunk, (int64_t)ncols);
+        if (hi > 0 && !wait_ctl(A, ctl, kSymAvail, (uint32_t)(hi - 1), sym_avail)) return false;
+        const int64_t ja = (int64_t)c * kChunk - (kWave - 1) + lane, jb = ja + kWave;
+        sa = (ja >= 0 && ja < ncols) ? (int32_t)L.sym[ja & (kSymRing - 1)] : 0;
+        sb = (lane < kStage - kWave && jb >= 0 && jb < ncols) ? (int32_t)L.sym[jb & (kSymRing - 1)] : 0;
+        return true;
+    };
+    auto stage_chunk = [&](int c, lds_u32 *buf, int32_t sa, int32_t sb) {
+        const int64_t ja = (int64_t)c * kChunk - (kWave - 1) + lane, jb = ja + kWave;
+        // table address of row code, lane l, minus 4t for t = p + l - 63:
+        // lane l reads position t - l + 63 at step t and adds 4t
+        const int32_t ca = code_of(sa, ja >= 0 && ja < ncols, sigma, bad);
+        buf[lane] = (uint32_t)((int32_t)tab_addr + ca * (kWave * 4) + (kWave - 1) * 4 - 4 * lane);
+        if (lane < kStage - kWave) {
+            const int32_t cb = code_of(sb, jb >= 0 && jb < ncols, sigma, bad);
+            buf[kWave + lane] = (uint32_t)((int32_t)tab_addr + cb * (kWave * 4) + (kWave - 1) * 4 - 4 * (kWave + lane));
         }
-        unsigned long long win = 0;
-        if (has_in) {
-            uint32_t w_lo = 0;
-            if (q + 2 < nchunks) {
-                w_lo = wait_word(A, rin + q + 2, e_next, tag_in, ok);
-                if (!ok) return false;
-                if (q + 3 < nchunks) e_next = ld_relaxed(rin + q + 3);
-            }
-            win = ((unsigned long long)w_hi << 32) | w_lo;
-            w_hi = w_lo;
-        }
-        // the chunk's 32 masks: staged addresses, then the table reads
-        const lds_u32 *sp = stage + (kWave - 1) - lane;
-        uint32_t M[kChunk];
+    };
+    auto read_masks = [&](const lds_u32 *buf, uint32_t (&M)[kChunk]) {
+        const lds_u32 *sp = buf + (kWave - 1) - lane;
 #pragma unroll
         for (int t = 0; t < kChunk; ++t) M[t] = sp[t];
 #pragma unroll
         for (int t = 0; t < kChunk; ++t) M[t] = *((const lds_u32 *)(uintptr_t)M[t] + t);
-        uint32_t wout = 0;
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t) {
-            const unsigned long long cin = (cout << 1) | ((win >> (32 - t)) & 1ull);
-            step(V, M[t], cin, cout);
-            wout = (wout << 1) | (uint32_t)(cout >> 63);
-        }
-        if (has_out && lane == 0) st_relaxed(rout + q, tag_out | wout);
+    };
+
+    uint32_t V = 0xffffffffu, wout = 0;
+    unsigned long long carry = 0;
+    uint32_t MA[kChunk], MB[kChunk];
+    int32_t s0 = 0, s1 = 0;
+    if (nchunks > 0) {
+        if (!raw_syms(0, s0, s1)) return false;
+        stage_chunk(0, L.stage, s0, s1);
+        read_masks(L.stage, MA);
+        if (nchunks > 1 && !raw_syms(1, s0, s1)) return false;
     }
+    // chunk q: stage and read chunk q+1 (raw symbols of chunk q+2 ahead)
+    // while chunk q combs; Mc and Mn swap roles, two chunks per iteration.
+    // The band below's word q+2 holds the carry-out bits of symbols 32q ..
+    // 32q+31 (it collects each bit one step late).
+    auto do_chunk = [&](int q, uint32_t (&Mc)[kChunk], uint32_t (&Mn)[kChunk]) -> bool {
+        uint32_t win = 0;
+        if (has_in && q + 2 < nchunks) {
+            if (!wait_ctl(A, ctl, kInAvail, (uint32_t)q + 2, in_avail)) return false;
+            win = L.in[(q + 2) & (kInRing - 1)];
+            if ((q & 15) == 15 && lane == 0) ctl[kInUsed] = (uint32_t)q + 3;
+        }
+        if (q + 1 < nchunks) {
+            lds_u32 *nb = L.stage + ((q + 1) & 1) * kStageWords;
+            stage_chunk(q + 1, nb, s0, s1);
+            if (q + 2 < nchunks && !raw_syms(q + 2, s0, s1)) return false;
+            if ((q & 15) == 15 && lane == 0) ctl[kSymUsed] = (uint32_t)max(0, (q + 3) * kChunk - (kWave - 1));
+            read_masks(nb, Mn);
+        }
+        chunk_steps(V, Mc, win, wout, carry);
+        if (has_out) {
+            if (q >= kOutRing && !wait_ctl(A, ctl, kOutUsed, (uint32_t)(q - kOutRing), out_used)) return false;
+            if (lane == 0) {
+                L.out[q & (kOutRing - 1)] = wout;
+                ctl[kOutAvail] = (uint32_t)q + 1;   // after the word: one wave's LDS writes land in order
+            }
+        }
+        return true;
+    };
+    int q = 0;
+    for (; q + 1 < nchunks && ok; q += 2) {
+        if (!do_chunk(q, MA, MB)) return false;
+        if (!do_chunk(q + 1, MB, MA)) return false;
+    }
+    if (q < nchunks && !do_chunk(q, MA, MB)) return false;
 
     if (__any(bad) && lane == 0) *D.err = 1;
     const int64_t live = D.nrows - r0;
@@ -216,21 +292,142 @@ __device__ bool band_llcs(const LlcsArgs &A, const LlcsDesc &D, int g, int lane,
     return true;
 }
 
-__global__ __launch_bounds__(kWave) void llcs_chain_kernel(LlcsArgs A)
+// Fetch wave of a band: all of the band's global memory traffic after its
+// table is built.  Each round it
+//  * loads the next column symbols (up to kSymBatch, 16 per lane) into the
+//    LDS symbol ring, as far as the comb wave's progress frees slots;
+//  * loads the band below's next 64 carry words from the global ring and
+//    copies the leading run that carries this band's tag to the LDS in-ring
+//    (one load brings every word published so far);
+//  * stores the comb wave's new carry words from the LDS out-ring to the
+//    global ring, tagged for the band above.
+// Loads are issued before any result is awaited, so a round costs about one
+// memory round trip.  Every wait is bounded in time.
+__device__ void band_fetch(const LlcsArgs &A, const LlcsDesc &D, int g, int lane, const Lds &L)
+{
+    lds_vu32 *ctl = L.ctl;
+    const int32_t ncols = D.ncols, nchunks = D.nchunks;
+    const bool need_in = g > 0 && nchunks > 2, need_out = g + 1 < D.nbands;
+    gcword *rin = (gcword *)(D.ring + (size_t)(g & 1) * nchunks);
+    gword *rout = (gword *)(D.ring + (size_t)((g + 1) & 1) * nchunks);
+    const uint32_t tag_in = (uint32_t)g;
+    const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
+    gsym *cols = (gsym *)D.cols;
+    int32_t next_s = 0, next_w = 2, out_done = 0;
+    unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        const bool done_s = next_s >= ncols, done_w = !need_in || next_w >= nchunks;
+        const bool done_o = !need_out || out_done >= nchunks;
+        if (done_s && done_w && done_o) return;
+        if (__builtin_amdgcn_readfirstlane(ctl[kStop])) return;
+        bool progress = false;
+
+        // issue: symbols, carry words in
+        int32_t scnt = 0;
+        int32_t sv[kSymBatch / kWave];
+        if (!done_s) {
+            const int32_t sym_used = (int32_t)__builtin_amdgcn_readfirstlane(ctl[kSymUsed]);
+            scnt = min(min(sym_used + kSymRing - next_s, kSymBatch), ncols - next_s);
+#pragma unroll
+            for (int k = 0; k < kSymBatch / kWave; ++k) {
+                const int32_t i = lane + k * kWave;
+                sv[k] = i < scnt ? cols[next_s + i] : 0;
+            }
+        }
+        int32_t want = 0;
+        unsigned long long e = 0;
+        if (!done_w) {
+            const int32_t used = (int32_t)__builtin_amdgcn_readfirstlane(ctl[kInUsed]);
+            want = min(min(used + kInRing - next_w, kWave), nchunks - next_w);
+            if (want > 0 && lane < want) e = ld_relaxed(rin + next_w + lane);
+        }
+        // carry words out
+        if (!done_o) {
+            const int32_t avail = (int32_t)__builtin_amdgcn_readfirstlane(ctl[kOutAvail]);
+            const int32_t k = min(avail - out_done, kWave);
+            if (k > 0) {
+                if (lane < k) st_relaxed(rout + out_done + lane, tag_out | L.out[(out_done + lane) & (kOutRing - 1)]);
+                out_done += k;
+                if (lane == 0) ctl[kOutUsed] = (uint32_t)out_done;   // after the reads of those slots
+                progress = true;
+            }
+        }
+        // land the loads
+        if (scnt > 0) {
+#pragma unroll
+            for (int k = 0; k < kSymBatch / kWave; ++k) {
+                const int32_t i = lane + k * kWave;
+                if (i < scnt) L.sym[(next_s + i) & (kSymRing - 1)] = (uint32_t)sv[k];
+            }
+            next_s += scnt;
+            if (lane == 0) ctl[kSymAvail] = (uint32_t)next_s;   // after the symbols
+            progress = true;
+        }
+        if (want > 0) {
+            const unsigned long long fresh = __ballot(lane < want && (uint32_t)(e >> 32) == tag_in);
+            const int k = ~fresh ? (int)__builtin_ctzll(~fresh) : kWave;   // leading run of fresh words
+            if (k > 0) {
+                if (lane < k) L.in[(next_w + lane) & (kInRing - 1)] = (uint32_t)e;
+                next_w += k;
+                if (lane == 0) ctl[kInAvail] = (uint32_t)next_w;   // after the words
+                progress = true;
+            }
+        }
+        if (progress) {
+            t0 = __builtin_amdgcn_s_memrealtime();
+        } else {
+            if (__builtin_amdgcn_s_memrealtime() - t0 > A.timeout_ticks) {
+                __hip_atomic_store(A.abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) ctl[kStop] = 1u;
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// One workgroup of two waves per band: wave 0 combs, wave 1 fetches.  Thread
+// 0 takes the ticket and publishes it (and whether to go on) in LDS; the
+// barriers keep both waves on the same band.
+__global__ __launch_bounds__(2 * kWave) void llcs_chain_kernel(LlcsArgs A)
 {
     extern __shared__ uint32_t lds_dyn[];
-    lds_u32 *stage = (lds_u32 *)lds_dyn;
-    lds_u32 *tab = stage + kStageWords + kZeroRowWords;
-    const int lane = threadIdx.x;
+    Lds L;
+    L.stage = (lds_u32 *)lds_dyn;
+    L.tab = L.stage + 2 * kStageWords + kZeroRowWords;
+    L.sym = L.tab + A.sigma * kWave;
+    L.in = L.sym + kSymRing;
+    L.out = L.in + kInRing;
+    L.ctl = (lds_vu32 *)(L.out + kOutRing);
+    lds_vu32 *ctl = L.ctl;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
     for (;;) {
-        unsigned t = 0;
-        if (lane == 0) t = atomicAdd(A.ticket, 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
-        if ((int)t >= A.ntasks) break;
-        if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) break;
+        if (threadIdx.x == 0) {
+            const unsigned t = atomicAdd(A.ticket, 1u);
+            ctl[kTask] = t;
+            ctl[kGo] = ((int)t < A.ntasks && !ld_flag(A.abort_flag)) ? 1u : 0u;
+            ctl[kStop] = 0u;
+            ctl[kInAvail] = 2u;
+            ctl[kInUsed] = 2u;
+            ctl[kSymAvail] = 0u;
+            ctl[kSymUsed] = 0u;
+            ctl[kOutAvail] = 0u;
+            ctl[kOutUsed] = 0u;
+        }
+        __syncthreads();
+        if (!__builtin_amdgcn_readfirstlane(ctl[kGo])) break;
+        const unsigned t = __builtin_amdgcn_readfirstlane(ctl[kTask]);
         const int2 task = A.tasks[t];
         const LlcsDesc D = A.descs[task.x];
-        if (!__builtin_amdgcn_readfirstlane((int)band_llcs(A, D, task.y, lane, stage, tab))) break;
+        if (wave == 0) {
+            const bool traced = A.trace && (int)t < A.trace_cap;
+            if (traced && lane == 0) A.trace[2 * (size_t)t] = __builtin_amdgcn_s_memrealtime();
+            if (!band_compute(A, D, task.y, lane, L) && lane == 0) ctl[kStop] = 1u;
+            if (traced && lane == 0) A.trace[2 * (size_t)t + 1] = __builtin_amdgcn_s_memrealtime();
+        } else {
+            band_fetch(A, D, task.y, lane, L);
+        }
+        __syncthreads();
     }
 }
 
@@ -270,7 +467,7 @@ int lcsl_launch(const lcsl::LlcsArgs &args, const lcsl::LlcsDesc *d_descs, int n
     hipLaunchKernelGGL(llcs_prep_kernel, dim3(px > 0 ? px : 1, npairs), dim3(256), 0, stream, d_descs, args.ticket,
                        args.abort_flag);
     if (args.ntasks > 0 && grid_blocks > 0)
-        hipLaunchKernelGGL(llcs_chain_kernel, dim3(grid_blocks), dim3(kWave), llcs_lds_bytes(args.sigma), stream,
+        hipLaunchKernelGGL(llcs_chain_kernel, dim3(grid_blocks), dim3(2 * kWave), llcs_lds_bytes(args.sigma), stream,
                            args);
     hipLaunchKernelGGL(llcs_finish_kernel, dim3((npairs + 255) / 256), dim3(256), 0, stream, d_descs, npairs);
     return hipGetLastError() == hipSuccess ? 0 : -5;
