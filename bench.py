@@ -69,9 +69,30 @@ def cpu_baseline(a, b, target_s: float):
     t0 = time.perf_counter()
     oracle.comb(a, b[:cols], order="mt", threads=threads)
     dt = time.perf_counter() - t0
+    # one core: the sequential restatement on a narrower full-height strip
+    cols1 = max(64, cols // (4 * threads))
+    t1 = time.perf_counter()
+    oracle.comb(a, b[:cols1], order="rows")
+    dt1 = time.perf_counter() - t1
     return {"value": round(m * cols / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
             "sample": f"m={m} x first {cols} columns of the same pair ({m * cols:.3g} cells, {dt:.1f} s), "
-                      f"oracle orc_comb_mt block wavefront, {threads} threads"}
+                      f"oracle orc_comb_mt block wavefront, {threads} threads",
+            "single_core": {"value": round(m * cols1 / dt1 / 1e9, 4), "unit": "GCUPS", "cores": 1,
+                            "sample": f"m={m} x first {cols1} columns ({m * cols1:.3g} cells, {dt1:.1f} s), "
+                                      "oracle orc_comb_rows (sequential restatement of lcs_reference.hpp:5-69)"}}
+
+
+def read_profile_value(path: str, prefix: str):
+    """A number after `prefix` on a line of a committed profile summary."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if prefix in line:
+                    tail = line.split(prefix, 1)[1].strip()
+                    return float(tail.split()[0])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
 
 
 def read_traffic(path: str):
@@ -383,7 +404,15 @@ def main():
     k_ms = float(np.mean(kernel_ms))
     peak_ops = cus * 64 * PEAK_CLOCK_HZ
     achieved_ops = OPS_PER_CELL * cells / (k_ms / 1e3)
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "comb_chain_mutated2^20_pmc.txt"))
+    pmc_file = os.path.join(ROOT, "profiles", "comb_chain_mutated2^20_pmc.txt")
+    traffic = read_traffic(pmc_file)
+    clock_ghz = read_profile_value(pmc_file, "GRBM_GUI_ACTIVE / 8 XCDs / kernel time =")
+    # end to end, as the reference times it: host strings in, permutation in host memory
+    t0 = time.perf_counter()
+    h2, v2, _ = ctx.comb(a, b, sched)
+    lcs_amd.from_strands(h2, v2)
+    e2e = time.perf_counter() - t0
+    e2e_same = bool(np.array_equal(h2, h) and np.array_equal(v2, v))
     out = {
         "metric": "GCUPS at m=n=2^20 (seaweed permutation, bit-exact)",
         "value": round(gcups, 2),
@@ -402,7 +431,12 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 3), "peak": round(peak_ops / 1e12, 3),
                      "unit": "TOP/s (int32 lane-ops, 4 per cell)", "frac": round(achieved_ops / peak_ops, 4),
                      "traffic": traffic, "kernel_ms": round(k_ms, 3),
-                     "kernel_gcups": round(cells / (k_ms / 1e3) / 1e9, 2)},
+                     "kernel_gcups": round(cells / (k_ms / 1e3) / 1e9, 2),
+                     "measured_clock_ghz": clock_ghz,
+                     "frac_at_measured_clock": (round(achieved_ops / (cus * 64 * clock_ghz * 1e9), 4)
+                                                if clock_ghz else None)},
+        "end_to_end": {"what": "lcs_comb on host strings + PermutationMatrix::FromStrands on the host",
+                       "ms": round(e2e * 1e3, 3), "gcups": round(cells / e2e / 1e9, 2), "same_strands": e2e_same},
     }
     if checked is not None:
         out["cross_schedule_bit_exact"] = checked
