@@ -132,20 +132,28 @@ typedef void LcsProcedure(LcsProblemContext &ctx);
 
 struct SemiLocalLcsImpl {
     bool ok = false;
+    bool score_only = false;   // the procedure fills length_of_lcs only (no strands)
     lcs_ctx *queue = nullptr;
     LcsFunction *func = nullptr;
     LcsProcedure *proc = nullptr;
     lcs_schedule schedule{};
     PermutationMatrix operator()(const LcsInput &input);
+    // Extension (the reference returns permutations only): the procedure's
+    // length_of_lcs, -1 on failure.  Works for every name; the score-only
+    // ones compute nothing else.
+    int length(const LcsInput &input);
 };
 
 // Names: "hip_auto" (schedule chosen by the engine) and
 // "hip_tiled_<tile_m>_<waves_per_simd>_<strips>" for tile_m in
-// {1,2,4,8,16,32}, waves_per_simd in {1,2}, strips in {1,2,4,8,16}.
+// {1,2,4,8,16,32}, waves_per_simd in {1,2}, strips in {1,2,4,8,16};
+// "hip_llcs": score only (length_of_lcs), the bit-parallel kernel when every
+// symbol lies in [0, 200), else the comb's read-out.
 struct LcsAlgorithmRegistry {
     std::map<std::string, LcsFunction *> reg;
     std::map<std::string, LcsProcedure *> reg2;
     std::map<std::string, lcs_schedule> schedules;
+    std::map<std::string, bool> score_only;
     lcs_ctx *QUEUE_GPU = nullptr;
 
     lcs_ctx *GetQueueForDeviceType(const std::string &device_type);
@@ -162,5 +170,9 @@ struct LcsAlgorithmRegistry {
 void Lcs_Semi_Hip(LcsProblemContext &ctx);
 // LcsFunction form (legacy signature of src/lcs_types.hpp:55).
 void Lcs_Semi_Hip_Function(const LcsInput &input, LcsContext &ctx);
+// Score-only procedure ("hip_llcs"): sets length_of_lcs, leaves the strands
+// unset, as the reference's unregistered score routines do
+// (Lcs_Prefix_Binary_Reference, src/lcs_reference.hpp:135-211).
+void Lcs_Llcs_Hip(LcsProblemContext &ctx);
 
 }  // namespace lcsreg

@@ -10,8 +10,9 @@
 // when not given, or whitespace-separated integers read from file_a/file_b
 // when given (the reference parses those keys but never reads the files).
 // Output: the reference's CSV columns, speed in cells/us (MCUPS), hash of
-// the permutation.  Blank lines are skipped (the reference's parser does
-// not advance past them).
+// the permutation; for score-only algorithms (hip_llcs), which produce no
+// permutation, the hash column holds the LCS length.  Blank lines are
+// skipped (the reference's parser does not advance past them).
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
@@ -147,9 +148,22 @@ int main(int argc, char **argv)
         lcsreg::LcsInput input(a.data(), (int)a.size(), b.data(), (int)b.size());
         for (int it = 0; it < o.iterations; ++it) {
             const auto t0 = std::chrono::steady_clock::now();
-            lcsreg::PermutationMatrix perm = impl(input);
+            lcsreg::PermutationMatrix perm;
+            int64_t result = 0;
+            if (impl.score_only) {
+                result = impl.length(input);   // score-only names: the hash column holds the LCS length
+            } else {
+                perm = impl(input);
+                result = perm.hash();
+            }
             const auto t1 = std::chrono::steady_clock::now();
-            if (perm.empty() && a.size() + b.size() > 0) {
+            if (impl.score_only && result < 0) {
+                std::fprintf(stderr, "Error: algorithm '%s' on '%s' produced no length\n", o.algorithm.c_str(),
+                             o.device_type.c_str());
+                ++failures;
+                continue;
+            }
+            if (!impl.score_only && perm.empty() && a.size() + b.size() > 0) {
                 std::fprintf(stderr, "Error: algorithm '%s' on '%s' produced no permutation\n", o.algorithm.c_str(),
                              o.device_type.c_str());
                 ++failures;
@@ -163,7 +177,7 @@ int main(int argc, char **argv)
             else std::fprintf(out, "%s(%d),", o.rng_a.c_str(), o.seed_a);
             if (!o.file_b.empty()) std::fprintf(out, "%s,", o.file_b.c_str());
             else std::fprintf(out, "%s(%d),", o.rng_b.c_str(), o.seed_b);
-            std::fprintf(out, "%f,%" PRId64 ",%f\n", ms, (int64_t)perm.hash(), speed);
+            std::fprintf(out, "%f,%" PRId64 ",%f\n", ms, result, speed);
         }
         std::fflush(out);
     }

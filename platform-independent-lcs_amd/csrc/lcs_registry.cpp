@@ -131,7 +131,48 @@ void Lcs_Semi_Hip_Function(const LcsInput &input, LcsContext &ctx)
     p.v_strands = nullptr;
 }
 
+void Lcs_Llcs_Hip(LcsProblemContext &ctx)
+{
+    lcs_ctx *gpu = ctx.queue;
+    if (!gpu || ctx.a_length < 0 || ctx.b_length < 0) return;
+    const int32_t *a = reinterpret_cast<const int32_t *>(ctx.a_given);
+    const int32_t *b = reinterpret_cast<const int32_t *>(ctx.b_given);
+    const int32_t m = ctx.a_length, n = ctx.b_length;
+    // the bit-parallel kernel takes symbols in [0, sigma), sigma <= 200
+    int32_t lo = 0, hi = 0;
+    for (int32_t i = 0; i < m; ++i) lo = std::min(lo, a[i]), hi = std::max(hi, a[i]);
+    for (int32_t j = 0; j < n; ++j) lo = std::min(lo, b[j]), hi = std::max(hi, b[j]);
+    int32_t llcs = -1;
+    if (lo >= 0 && hi < 200) {
+        if (lcs_llcs(gpu, a, m, b, n, hi + 1, &llcs) == LCS_OK) ctx.length_of_lcs = llcs;
+        return;
+    }
+    std::vector<int32_t> h((size_t)std::max(m, 1)), v((size_t)std::max(n, 1));
+    if (lcs_comb(gpu, a, m, b, n, nullptr, h.data(), v.data(), &llcs) == LCS_OK) ctx.length_of_lcs = llcs;
+}
+
 // ------------------------------------------------------------ registry
+
+int SemiLocalLcsImpl::length(const LcsInput &input)
+{
+    if (func) {
+        LcsContext ctx(queue);
+        func(input, ctx);
+        return ctx.llcs;
+    }
+    if (proc) {
+        LcsProblemContext p;
+        p.a_given = reinterpret_cast<const Word *>(input.a_data);
+        p.a_length = input.a_size;
+        p.b_given = reinterpret_cast<const Word *>(input.b_data);
+        p.b_length = input.b_size;
+        p.queue = queue;
+        p.schedule = schedule;
+        proc(p);
+        return p.length_of_lcs;
+    }
+    return -1;
+}
 
 PermutationMatrix SemiLocalLcsImpl::operator()(const LcsInput &input)
 {
@@ -164,6 +205,8 @@ LcsAlgorithmRegistry::LcsAlgorithmRegistry()
     reg2["hip_auto"] = Lcs_Semi_Hip;
     schedules["hip_auto"] = lcs_schedule{0, 0, 0, 0};
     reg["hip_auto_fn"] = Lcs_Semi_Hip_Function;
+    reg2["hip_llcs"] = Lcs_Llcs_Hip;
+    score_only["hip_llcs"] = true;
     const int tiles[] = {1, 2, 4, 8, 16, 32};
     const int strips[] = {1, 2, 4, 8, 16};
     for (int tm : tiles)
@@ -203,6 +246,7 @@ SemiLocalLcsImpl LcsAlgorithmRegistry::Get(const std::string &name, const std::s
         result.queue = GetQueueForDeviceType(device_type);
         result.proc = reg2[name];
         result.schedule = schedules[name];
+        result.score_only = score_only.count(name) > 0;
         result.ok = true;  // as in the reference: a known name is ok, the queue may still be null
         return result;
     }

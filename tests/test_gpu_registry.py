@@ -63,3 +63,32 @@ def test_cli_reads_input_files(tmp_path):
     row = next(csv.DictReader(io.StringIO(out.read_text())))
     h, v = oracle.comb(a, b)
     assert int(row["hash"]) == oracle.perm_hash(oracle.from_strands(h, v)[1])
+
+
+def test_cli_score_only_llcs(tmp_path):
+    """hip_llcs: the hash column holds the LCS length; binary script inputs
+    against the golden fixtures, a 26-symbol file pair (bit-parallel path)
+    and a pair with symbols outside [0, 200) (the comb read-out) against the
+    DP."""
+    import numpy as np
+    import oracle
+    rng = np.random.default_rng(3)
+    a26, b26 = rng.integers(0, 26, 2500), rng.integers(0, 26, 3100)
+    aw, bw = rng.integers(-5, 1000, 1200), rng.integers(-5, 1000, 900)
+    for name, x in (("a26", a26), ("b26", b26), ("aw", aw), ("bw", bw)):
+        (tmp_path / f"{name}.txt").write_text(" ".join(map(str, x)))
+    script = tmp_path / "s.txt"
+    script.write_text(
+        "algorithm=hip_llcs device_type=gpu size_a=7330 size_b=9944 seed_a=888 seed_b=999\n"
+        "algorithm=hip_llcs device_type=gpu size_a=9771 size_b=11302 seed_a=888 seed_b=999\n"
+        f"algorithm=hip_llcs device_type=gpu file_a={tmp_path / 'a26.txt'} file_b={tmp_path / 'b26.txt'}\n"
+        f"algorithm=hip_llcs device_type=gpu file_a={tmp_path / 'aw.txt'} file_b={tmp_path / 'bw.txt'}\n")
+    out = tmp_path / "o.csv"
+    r = subprocess.run([CLI, str(script), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = list(csv.DictReader(io.StringIO(out.read_text())))
+    g = golden()
+    assert int(rows[0]["hash"]) == g[(7330, 9944, 888, 999)]["llcs"]
+    assert int(rows[1]["hash"]) == g[(9771, 11302, 888, 999)]["llcs"]
+    assert int(rows[2]["hash"]) == oracle.llcs_dp(a26, b26)
+    assert int(rows[3]["hash"]) == oracle.llcs_dp(aw, bw)
