@@ -121,14 +121,9 @@ __device__ __forceinline__ bool wait_ctl(const LlcsArgs &A, lds_vu32 *ctl, int w
     : [v] "+v"(V), [o] "+s"(wout), [c] "+s"(carry), [u] "=&v"(u), [s] "=&v"(sum)                       \
     : [w] "s"(win), [m0] "v"(M[OFF + 0]), [m1] "v"(M[OFF + 1]), [m2] "v"(M[OFF + 2]),                  \
       [m3] "v"(M[OFF + 3]), [m4] "v"(M[OFF + 4]), [m5] "v"(M[OFF + 5]), [m6] "v"(M[OFF + 6]),           \
-      [m7] "v"(M[OFF + 7]), [m8] "v"(M[OFF + 8]), [m9] "v"(M[OFF + 9]), [m10] "v"(M[OFF + 10]),         \
-      [m11] "v"(M[OFF + 11]), [m12] "v"(M[OFF + 12]), [m13] "v"(M[OFF + 13]), [m14] "v"(M[OFF + 14]),   \
-      [m15] "v"(M[OFF + 15])                                                                            \
+      [m7] "v"(M[OFF + 7])                                                                              \
     : "vcc", "scc"
 
-// The 32 steps of a chunk: carry-in bits come from w (bit 31 - t at step t),
-// carry-out bits of steps t-1 are collected into wout (the previous chunk's
-// last one first).
 // (readfirstlane returns int: each half goes through uint32_t so the low
 // half is not sign-extended into the high one)
 __device__ __forceinline__ uint64_t uniform64(uint64_t x)
@@ -138,29 +133,41 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t x)
     return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ void chunk_steps(uint32_t &V, const uint32_t (&M)[kChunk], uint32_t win, uint32_t &wout,
-                                            unsigned long long &carry)
+// Eight steps (quarter Q of a chunk): carry-in bits come from w (bit 31 - t
+// at step t of the chunk), carry-out bits of the previous step are
+// collected into wout.  Volatile, so the LDS reads the chunk loop places
+// between quarters stay there: at most a quarter's reads are outstanding at
+// each block, well under lgkmcnt's limit of 15 (a wait for an older read
+// then never waits for the newer ones).
+template <int Q>
+__device__ __forceinline__ void quarter_steps(uint32_t &V, const uint32_t (&M)[kChunk], uint32_t win, uint32_t &wout,
+                                              unsigned long long &carry)
 {
-    // all three are wave-uniform by construction; readfirstlane lets the
-    // compiler see it across the early exits of the chunk loop
-    win = __builtin_amdgcn_readfirstlane(win);
-    wout = __builtin_amdgcn_readfirstlane(wout);
-    carry = uniform64(carry);
     uint32_t u, sum;
-    asm("s_mov_b64 vcc, %[c]\n\t"
-        LCSL_STEP(31, 0) LCSL_STEP(30, 1) LCSL_STEP(29, 2) LCSL_STEP(28, 3)
-        LCSL_STEP(27, 4) LCSL_STEP(26, 5) LCSL_STEP(25, 6) LCSL_STEP(24, 7)
-        LCSL_STEP(23, 8) LCSL_STEP(22, 9) LCSL_STEP(21, 10) LCSL_STEP(20, 11)
-        LCSL_STEP(19, 12) LCSL_STEP(18, 13) LCSL_STEP(17, 14) LCSL_STEP(16, 15)
-        "s_mov_b64 %[c], vcc"
-        LCSL_OPERANDS(M, 0));
-    asm("s_mov_b64 vcc, %[c]\n\t"
-        LCSL_STEP(15, 0) LCSL_STEP(14, 1) LCSL_STEP(13, 2) LCSL_STEP(12, 3)
-        LCSL_STEP(11, 4) LCSL_STEP(10, 5) LCSL_STEP(9, 6) LCSL_STEP(8, 7)
-        LCSL_STEP(7, 8) LCSL_STEP(6, 9) LCSL_STEP(5, 10) LCSL_STEP(4, 11)
-        LCSL_STEP(3, 12) LCSL_STEP(2, 13) LCSL_STEP(1, 14) LCSL_STEP(0, 15)
-        "s_mov_b64 %[c], vcc"
-        LCSL_OPERANDS(M, 16));
+    if constexpr (Q == 0)
+        asm volatile("s_mov_b64 vcc, %[c]\n\t"
+                     LCSL_STEP(31, 0) LCSL_STEP(30, 1) LCSL_STEP(29, 2) LCSL_STEP(28, 3)
+                     LCSL_STEP(27, 4) LCSL_STEP(26, 5) LCSL_STEP(25, 6) LCSL_STEP(24, 7)
+                     "s_mov_b64 %[c], vcc"
+                     LCSL_OPERANDS(M, 0));
+    else if constexpr (Q == 1)
+        asm volatile("s_mov_b64 vcc, %[c]\n\t"
+                     LCSL_STEP(23, 0) LCSL_STEP(22, 1) LCSL_STEP(21, 2) LCSL_STEP(20, 3)
+                     LCSL_STEP(19, 4) LCSL_STEP(18, 5) LCSL_STEP(17, 6) LCSL_STEP(16, 7)
+                     "s_mov_b64 %[c], vcc"
+                     LCSL_OPERANDS(M, 8));
+    else if constexpr (Q == 2)
+        asm volatile("s_mov_b64 vcc, %[c]\n\t"
+                     LCSL_STEP(15, 0) LCSL_STEP(14, 1) LCSL_STEP(13, 2) LCSL_STEP(12, 3)
+                     LCSL_STEP(11, 4) LCSL_STEP(10, 5) LCSL_STEP(9, 6) LCSL_STEP(8, 7)
+                     "s_mov_b64 %[c], vcc"
+                     LCSL_OPERANDS(M, 16));
+    else
+        asm volatile("s_mov_b64 vcc, %[c]\n\t"
+                     LCSL_STEP(7, 0) LCSL_STEP(6, 1) LCSL_STEP(5, 2) LCSL_STEP(4, 3)
+                     LCSL_STEP(3, 4) LCSL_STEP(2, 5) LCSL_STEP(1, 6) LCSL_STEP(0, 7)
+                     "s_mov_b64 %[c], vcc"
+                     LCSL_OPERANDS(M, 24));
 }
 
 // Column symbol j as a table row: [0, sigma) as is, -1 (the zero row) for
@@ -215,8 +222,10 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
         const int64_t hi = min((int64_t)c * kChunk + kChunk, (int64_t)ncols);
         if (hi > 0 && !wait_ctl(A, ctl, kSymAvail, (uint32_t)(hi - 1), sym_avail)) return false;
         const int64_t ja = (int64_t)c * kChunk - (kWave - 1) + lane, jb = ja + kWave;
-        sa = (ja >= 0 && ja < ncols) ? (int32_t)L.sym[ja & (kSymRing - 1)] : 0;
-        sb = (lane < kStage - kWave && jb >= 0 && jb < ncols) ? (int32_t)L.sym[jb & (kSymRing - 1)] : 0;
+        // unconditional reads (every index is a ring slot); stage_chunk
+        // ignores the values of positions outside the string
+        sa = (int32_t)L.sym[ja & (kSymRing - 1)];
+        sb = (int32_t)L.sym[jb & (kSymRing - 1)];
         return true;
     };
     auto stage_chunk = [&](int c, lds_u32 *buf, int32_t sa, int32_t sb) {
@@ -230,12 +239,26 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
             buf[kWave + lane] = (uint32_t)((int32_t)tab_addr + cb * (kWave * 4) + (kWave - 1) * 4 - 4 * (kWave + lane));
         }
     };
-    auto read_masks = [&](const lds_u32 *buf, uint32_t (&M)[kChunk]) {
+    // staged addresses of a chunk (32 reads, 16 ds_read2), then its masks
+    // in quarters of 8 dependent reads
+    auto read_addrs = [&](const lds_u32 *buf, uint32_t (&M)[kChunk]) {
         const lds_u32 *sp = buf + (kWave - 1) - lane;
 #pragma unroll
         for (int t = 0; t < kChunk; ++t) M[t] = sp[t];
+    };
+    auto read_quarter = [&](uint32_t (&M)[kChunk], int qtr) {
 #pragma unroll
-        for (int t = 0; t < kChunk; ++t) M[t] = *((const lds_u32 *)(uintptr_t)M[t] + t);
+        for (int i = 0; i < kChunk / 4; ++i) {
+            const int t = qtr * (kChunk / 4) + i;
+            M[t] = *((const lds_u32 *)(uintptr_t)M[t] + t);
+        }
+    };
+    auto read_masks = [&](const lds_u32 *buf, uint32_t (&M)[kChunk]) {
+        read_addrs(buf, M);
+        read_quarter(M, 0);
+        read_quarter(M, 1);
+        read_quarter(M, 2);
+        read_quarter(M, 3);
     };
 
     uint32_t V = 0xffffffffu, wout = 0;
@@ -252,21 +275,46 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
     // while chunk q combs; Mc and Mn swap roles, two chunks per iteration.
     // The band below's word q+2 holds the carry-out bits of symbols 32q ..
     // 32q+31 (it collects each bit one step late).
+    // the band below's word for chunk q is read during chunk q-1 (before
+    // that chunk's last quarter of mask reads, so waiting for it never waits
+    // for them)
+    // (the read itself is unconditional and masked only where the chunk
+    // uses it, so that waiting for it never waits for later reads)
+    auto ring_word = [&](int q, uint32_t &w, bool &use) -> bool {
+        use = has_in && q + 2 < nchunks;
+        if (use && !wait_ctl(A, ctl, kInAvail, (uint32_t)q + 2, in_avail)) return false;
+        w = L.in[(q + 2) & (kInRing - 1)];
+        if (use && (q & 15) == 15 && lane == 0) ctl[kInUsed] = (uint32_t)q + 3;
+        return true;
+    };
+    uint32_t win_next = 0;
+    bool use_next = false;
+    if (!ring_word(0, win_next, use_next)) return false;
     auto do_chunk = [&](int q, uint32_t (&Mc)[kChunk], uint32_t (&Mn)[kChunk]) -> bool {
-        uint32_t win = 0;
-        if (has_in && q + 2 < nchunks) {
-            if (!wait_ctl(A, ctl, kInAvail, (uint32_t)q + 2, in_avail)) return false;
-            win = L.in[(q + 2) & (kInRing - 1)];
-            if ((q & 15) == 15 && lane == 0) ctl[kInUsed] = (uint32_t)q + 3;
-        }
-        if (q + 1 < nchunks) {
-            lds_u32 *nb = L.stage + ((q + 1) & 1) * kStageWords;
-            stage_chunk(q + 1, nb, s0, s1);
-            if (q + 2 < nchunks && !raw_syms(q + 2, s0, s1)) return false;
-            if ((q & 15) == 15 && lane == 0) ctl[kSymUsed] = (uint32_t)max(0, (q + 3) * kChunk - (kWave - 1));
-            read_masks(nb, Mn);
-        }
-        chunk_steps(V, Mc, win, wout, carry);
+        uint32_t win = use_next ? win_next : 0u;
+        // chunk q combs in four quarters; between them the next chunk's
+        // masks are read a quarter at a time (addresses first).  After the
+        // last chunk this stages chunk nchunks, all padding (zero-row
+        // addresses), and reads nothing else.
+        lds_u32 *nb = L.stage + ((q + 1) & 1) * kStageWords;
+        stage_chunk(q + 1, nb, s0, s1);
+        read_addrs(nb, Mn);
+        // wave-uniform by construction; readfirstlane lets the compiler see
+        // it across the early exits of the chunk loop
+        win = __builtin_amdgcn_readfirstlane(win);
+        wout = __builtin_amdgcn_readfirstlane(wout);
+        carry = uniform64(carry);
+        quarter_steps<0>(V, Mc, win, wout, carry);
+        read_quarter(Mn, 0);
+        quarter_steps<1>(V, Mc, win, wout, carry);
+        read_quarter(Mn, 1);
+        quarter_steps<2>(V, Mc, win, wout, carry);
+        read_quarter(Mn, 2);
+        if (!raw_syms(q + 2, s0, s1)) return false;
+        if ((q & 15) == 15 && lane == 0) ctl[kSymUsed] = (uint32_t)max(0, (q + 3) * kChunk - (kWave - 1));
+        if (!ring_word(q + 1, win_next, use_next)) return false;
+        quarter_steps<3>(V, Mc, win, wout, carry);
+        read_quarter(Mn, 3);
         if (has_out) {
             if (q >= kOutRing && !wait_ctl(A, ctl, kOutUsed, (uint32_t)(q - kOutRing), out_used)) return false;
             if (lane == 0) {
