@@ -267,6 +267,8 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     int k = 0;
     for (; k < nchunks && !full(k); ++k)
         if (!chunk(Masked{}, k)) return false;
+    for (; k + 1 < nchunks && full(k + 1); k += 2)
+        if (!chunk(Full{}, k) || !chunk(Full{}, k + 1)) return false;
     for (; k < nchunks && full(k); ++k)
         if (!chunk(Full{}, k)) return false;
     for (; k < nchunks; ++k)
