@@ -205,18 +205,25 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     // (the first and last chunks of a band) go to trash words past the ring.
     static_assert(C > 0 && (C & (C - 1)) == 0 && C <= kWave, "C must be a power of two <= 64");
     gword *trash = (gword *)(P.E + 2 * (size_t)n);
-    auto publish = [&](uint32_t Ox, int t) {
+    // INSIDE: every column is known to be in [0, n), so the store needs no
+    // range check (the interior chunks below).
+    auto publish = [&](uint32_t Ox, int t, auto inside) {
         const int src = kWave - C + (lane & (C - 1));
         const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)Ox);
         const int col = t + C - 2 * kWave + src;
-        const bool mine = col >= 0 && col < n;
-        st_relaxed(mine ? Eout + col : trash + src, tag_out | (unsigned long long)x);
+        if constexpr (decltype(inside)::value) {
+            st_relaxed(Eout + (uint32_t)col, tag_out | (unsigned long long)x);
+        } else {
+            const bool mine = col >= 0 && col < n;
+            st_relaxed(mine ? Eout + col : trash + src, tag_out | (unsigned long long)x);
+        }
     };
     // Turns the prefetched ring words of the chunk starting at t into lane 0's
     // conveyor I (waiting, bounded in time, until they carry this band's tag).
-    auto take_inputs = [&](int t, uint32_t &I) -> bool {
+    // INSIDE: the chunk's columns are known to be < n.
+    auto take_inputs = [&](int t, uint32_t &I, auto inside) -> bool {
         const int c = t + (lane & (C - 1));   // lanes >= C repeat lanes < C
-        const bool need = lane < C && c < n;
+        const bool need = lane < C && (decltype(inside)::value || c < n);
         unsigned long long e = pE;
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
@@ -239,41 +246,52 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     };
     uint32_t v = 0, O = 0, Osave = 0, I = 0;
     int32_t bq = 0, IB = pB;
-    if (!take_inputs(0, I)) return false;
+    if (!take_inputs(0, I, std::false_type{})) return false;
     // One chunk: prefetch chunk k+1, then publish chunk k-1 (the wait for the
     // loads at the end then leaves the store in flight), comb C steps, take
-    // the next inputs.  The chunks run in three loops (ramp-in, full,
+    // the next inputs.  The chunks run in three phases (ramp-in, interior,
     // ramp-out) so the hot loop holds only the unmasked steps and no
-    // register copies between masked and unmasked paths.
+    // register copies between masked and unmasked paths; the interior loop
+    // takes two chunks per iteration, which lets the compiler close it
+    // without copying the h registers (with one chunk it copied them).
+    using Masked = std::integral_constant<bool, true>;
+    using Full = std::integral_constant<bool, false>;
     auto chunk = [&](auto colmask, int k) -> bool {
         const int t0 = k * C;
-        {
+        if constexpr (decltype(colmask)::value) {
             const int c = min(t0 + C + (lane & (C - 1)), nm1);
             pB = bsym[c];
             pE = ld_relaxed(Ein + c);
+        } else {   // an interior chunk: the next chunk's columns are inside [0, n)
+            const uint32_t c = (uint32_t)(t0 + C + (lane & (C - 1)));
+            pB = bsym[c];
+            pE = ld_relaxed(Ein + c);
         }
-        publish(Osave, t0 - C);
+        publish(Osave, t0 - C, std::integral_constant<bool, !decltype(colmask)::value>{});
         band_steps<TM, C, ROWMASK, decltype(colmask)::value>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
         Osave = O;
-        if (k + 1 < nchunks) {
+        if (!decltype(colmask)::value || k + 1 < nchunks) {   // an interior chunk is never the last
             IB = pB;
-            if (!take_inputs(t0 + C, I)) return false;
+            if (!take_inputs(t0 + C, I, std::integral_constant<bool, !decltype(colmask)::value>{})) return false;
         }
         return true;
     };
-    using Masked = std::integral_constant<bool, true>;
-    using Full = std::integral_constant<bool, false>;
-    auto full = [&](int k) { return k * C >= kWave - 1 && k * C + C <= n; };
+    // Interior chunks: no lane's column leaves [0, n) (t0 >= 63, t0 + C <= n),
+    // the next chunk's prefetch stays inside (t0 + 2C <= n) and so does the
+    // publish of the chunk before (columns t0 - 64 - C .. t0 - 65, so
+    // t0 >= 64 + C).
+    const int k_in = (kWave + C) / C;                      // first interior chunk
+    const int k_out = max(k_in, min(n / C - 1, nchunks));  // first chunk after them
     int k = 0;
-    for (; k < nchunks && !full(k); ++k)
+    for (; k < min(k_in, nchunks); ++k)
         if (!chunk(Masked{}, k)) return false;
-    for (; k + 1 < nchunks && full(k + 1); k += 2)
+    for (; k + 1 < k_out; k += 2)
         if (!chunk(Full{}, k) || !chunk(Full{}, k + 1)) return false;
-    for (; k < nchunks && full(k); ++k)
+    for (; k < k_out; ++k)
         if (!chunk(Full{}, k)) return false;
     for (; k < nchunks; ++k)
         if (!chunk(Masked{}, k)) return false;
-    publish(Osave, (nchunks - 1) * C);
+    publish(Osave, (nchunks - 1) * C, std::false_type{});
 #pragma unroll
     for (int k = 0; k < TM; ++k)
         if (valid[k]) P.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];

@@ -32,6 +32,21 @@ def test_edge_shapes(ctx, m, n):
         _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
 
 
+def test_interior_chunk_bounds(ctx):
+    """Widths around the band loop's phase boundaries: a chunk is 16 steps,
+    the interior (unmasked) chunks start at step 80 and stop two chunks
+    before n, and run two per iteration, so every n mod 32 and the widths
+    with zero, one and two interior chunks are covered, across several
+    bands."""
+    rng = np.random.default_rng(404)
+    for n in list(range(76, 84)) + [95, 96, 97, 111, 112, 113, 127, 128, 129, 143, 144, 145, 159, 160, 161,
+                                    175, 176, 177, 191, 192, 193, 207, 208, 209, 1000, 1001, 1023, 1024, 1025]:
+        a = rng.integers(0, 3, 2500).astype(np.int32)
+        b = rng.integers(0, 3, n).astype(np.int32)
+        for tm in (4, 16):
+            _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
+
+
 @pytest.mark.parametrize("tm", [1, 2, 4, 8, 16, 32])
 def test_tile_heights(ctx, tm):
     rng = np.random.default_rng(tm)
