@@ -46,6 +46,20 @@ constexpr int kLdsCoreK = 16384;
 constexpr int kMaxChunks = 128;
 constexpr size_t lds_core_bytes(int K) { return (size_t)4 * K * sizeof(uint16_t) + ((size_t)K / 64 + 1) * 4; }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP: four row_shr
+// steps scan each row of 16 lanes, then row_bcast:15 and row_bcast:31 carry
+// the row totals forward (lanes without a source add the old value, 0).
+__device__ __forceinline__ int wave_incl_scan(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // Exclusive prefix of the 0/1 flags of keys[0..K) (bit `bit`), written to
 // S[0..K] (S[K] = total).  Each thread owns a contiguous run of elements.
 __device__ void block_scan_bits(const int32_t *keys, int K, int bit, int32_t *S, int32_t *lds_sums)
@@ -120,6 +134,7 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     // chunk height: 32 rows, or s/128 for big nodes, balancing the
     // boundary scans (O(s) each) against the walks (O(rows) each)
     const int rc = s < kChunkRows ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
+    const int rc_log = __builtin_ctz(rc);   // rc and s are powers of two
     const int nch = s / rc;            // chunks per node
     const int total = K / rc;          // chunks at this depth
     // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc).
@@ -151,16 +166,12 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             for (int c0 = 0; c0 < s; c0 += 64) {
                 const uint16_t e = CI[v0 + c0 + lane];
-                if (e & kLoFlag) atomicAdd(&hlo[(s - 1 - (e & kIdx)) / rc], 1);
+                if (e & kLoFlag) atomicAdd(&hlo[(s - 1 - (e & kIdx)) >> rc_log], 1);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             int a0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             int a1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            int incl = a0 + a1;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int v = __shfl_up(incl, off);
-                if (lane >= off) incl += v;
-            }
+            const int incl = wave_incl_scan(a0 + a1);
             const int Y0 = incl - a0 - a1, Y1 = Y0 + a0;   // exclusive prefixes at j0, j1
             // sweep 2: running Z_j per boundary until Z_j reaches Y_j
             int Z0 = 0, Z1 = 0, T0 = 0, T1 = 0;
@@ -173,22 +184,15 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 const uint16_t e = CI[v0 + c0 + lane];
                 const bool lo = (e & kLoFlag) != 0;
-                const int bk = (s - 1 - (e & kIdx)) / rc;
+                const int bk = (s - 1 - (e & kIdx)) >> rc_log;
                 atomicAdd(lo ? &hlo[bk] : &hhi[bk], 1);
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 const int l0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 const int l1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 const int g0 = __hip_atomic_load(&hhi[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 const int g1 = __hip_atomic_load(&hhi[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-                int il = l0 + l1, ih = g0 + g1;
-                for (int off = 1; off < 64; off <<= 1) {
-                    const int vl = __shfl_up(il, off), vh = __shfl_up(ih, off);
-                    if (lane >= off) {
-                        il += vl;
-                        ih += vh;
-                    }
-                }
-                const int hi_total = __shfl(ih, 63);
+                const int il = wave_incl_scan(l0 + l1), ih = wave_incl_scan(g0 + g1);
+                const int hi_total = __builtin_amdgcn_readlane(ih, 63);
                 const int lp0 = il - l0 - l1, lp1 = lp0 + l0;      // lo columns with bucket < j
                 const int hp0 = ih - g0 - g1, hp1 = hp0 + g0;      // hi columns with bucket < j
                 const int cnt0 = lp0 + (hi_total - hp0), cnt1 = lp1 + (hi_total - hp1);
@@ -231,6 +235,12 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
         }
     }
     __syncthreads();
+    // The walk (the reference's ant passage, :123-189) with the four entries
+    // it reads kept in registers: cl = CI[dc-1], cc = CI[dc], ud = U[dr],
+    // ur = U[dr-1].  A step right only needs the new CI[dc] and a step down
+    // only the new U[dr-1], so every step is one LDS read from a selected
+    // address and no lane of the wave branches on the direction.
+    auto f_row = [](uint16_t e, int r) { return (e & kLoFlag) ? ((e & kIdx) >= r) : ((e & kIdx) < r); };
     for (int ch = threadIdx.x; ch < total; ch += kThreads) {
         const int v0 = (ch / nch) * s, j = ch % nch;
         const int r_hi = s - j * rc, r_lo = r_hi - rc;
@@ -243,26 +253,28 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
             dc = bT[ch];
             delta = bD[ch];
         }
+        // entries outside the node (dc = 0 or s, dr = 0) are read clamped and never used
+        uint16_t cl = CI[v0 + max(dc - 1, 0)], cc = CI[v0 + min(dc, s - 1)];
+        uint16_t ud = U[v0 + dr], ur = U[v0 + max(dr - 1, 0)];
         while (dr >= r_lo) {
             if (dc > 0) {
-                const uint16_t el = CI[v0 + dc - 1];
-                const int d_left = delta - ((el & kLoFlag) ? ((el & kIdx) >= dr) : ((el & kIdx) < dr));
-                const uint16_t ed = U[v0 + dr];
-                const int d_down = delta + ((ed & kLoFlag) ? ((ed & kIdx) >= dc) : ((ed & kIdx) < dc));
+                const int d_left = delta - f_row(cl, dr);
+                const int d_down = delta + f_row(ud, dc);
                 if (d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
             }
             // delta(dr, s) >= 0 always; the dc < s test only bounds the walk
-            if (delta < 0 && dc < s) {
-                const uint16_t ec = CI[v0 + dc];
-                delta += (ec & kLoFlag) ? ((ec & kIdx) >= dr) : ((ec & kIdx) < dr);
+            const bool right = delta < 0 && dc < s;
+            if (right) {
+                delta += f_row(cc, dr);
                 ++dc;
+                cl = cc;
             } else {
-                if (dr > 0) {
-                    const uint16_t er = U[v0 + dr - 1];
-                    delta -= (er & kLoFlag) ? ((er & kIdx) >= dc) : ((er & kIdx) < dc);
-                }
+                if (dr > 0) delta -= f_row(ur, dc);
                 --dr;
+                ud = ur;
             }
+            const uint16_t nx = right ? CI[v0 + min(dc, s - 1)] : U[v0 + max(dr - 1, 0)];
+            if (right) cc = nx; else ur = nx;
         }
     }
     __syncthreads();
@@ -330,13 +342,9 @@ __device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t 
         int carry = 0;
         for (int base = 0; base < ngroups; base += 64) {
             const int c = base + lane < ngroups ? gpre[base + lane] : 0;
-            int incl = c;
-            for (int off = 1; off < 64; off <<= 1) {
-                const int v = __shfl_up(incl, off);
-                if (lane >= off) incl += v;
-            }
+            const int incl = wave_incl_scan(c);
             if (base + lane < ngroups) gpre[base + lane] = carry + incl - c;
-            carry += __shfl(incl, 63);
+            carry += __builtin_amdgcn_readlane(incl, 63);
         }
         if (lane == 0) gpre[ngroups] = carry;
     }
