@@ -155,28 +155,40 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     if (nch > 1) {
         int *hlo = hist + wave * 2 * kMaxChunks, *hhi = hlo + kMaxChunks;   // this wave's counters
         const int j0 = 2 * lane, j1 = 2 * lane + 1;
-        for (int node = wave; node < K / s; node += kThreads / 64) {
-            const int v0 = node * s;
-            // sweep 1: Y_j = #{lo columns with bucket < j}
+        // Bucket counts of the columns [c_begin, c_end) of a node, as seen
+        // from boundaries j0 and j1: lo columns with bucket < j (lp) and
+        // columns with z_j (cnt).  The wave's LDS accesses run in order; the
+        // fences keep the compiler from moving counter stores, adds and loads
+        // across each other.
+        auto range_counts = [&](int v0, int c_begin, int c_end, int &lp0, int &lp1, int &cnt0, int &cnt1) {
             hlo[j0] = 0;
             hlo[j1] = 0;
-            // the wave's LDS accesses run in order; the fences keep the
-            // compiler from moving counter stores, adds and loads across
-            // each other
+            hhi[j0] = 0;
+            hhi[j1] = 0;
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            for (int c0 = 0; c0 < s; c0 += 64) {
+            for (int c0 = c_begin; c0 < c_end; c0 += 64) {
                 const uint16_t e = CI[v0 + c0 + lane];
-                if (e & kLoFlag) atomicAdd(&hlo[(s - 1 - (e & kIdx)) >> rc_log], 1);
+                const int bk = (s - 1 - (e & kIdx)) >> rc_log;
+                atomicAdd((e & kLoFlag) ? &hlo[bk] : &hhi[bk], 1);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            int a0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            int a1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            const int incl = wave_incl_scan(a0 + a1);
-            const int Y0 = incl - a0 - a1, Y1 = Y0 + a0;   // exclusive prefixes at j0, j1
-            // sweep 2: running Z_j per boundary until Z_j reaches Y_j
-            int Z0 = 0, Z1 = 0, T0 = 0, T1 = 0;
-            bool open0 = j0 >= 1 && j0 < nch && Y0 > 0, open1 = j1 < nch && Y1 > 0;
-            for (int c0 = 0; c0 < s && __ballot(open0 || open1); c0 += 64) {
+            const int l0 = __hip_atomic_load(&hlo[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const int l1 = __hip_atomic_load(&hlo[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const int g0 = __hip_atomic_load(&hhi[j0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const int g1 = __hip_atomic_load(&hhi[j1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const int il = wave_incl_scan(l0 + l1), ih = wave_incl_scan(g0 + g1);
+            const int hi_total = __builtin_amdgcn_readlane(ih, 63);
+            lp0 = il - l0 - l1;
+            lp1 = lp0 + l0;
+            const int hp0 = ih - g0 - g1, hp1 = hp0 + g0;   // hi columns with bucket < j
+            cnt0 = lp0 + (hi_total - hp0);
+            cnt1 = lp1 + (hi_total - hp1);
+        };
+        // Running Z_j over [c_begin, c_end) in rounds of 64 columns for the
+        // open boundaries until Z_j reaches Y_j; T = the column where it does.
+        auto find_T = [&](int v0, int c_begin, int c_end, int Y0, int Y1, int Z0, int Z1, bool open0, bool open1,
+                          int &T0, int &T1) {
+            for (int c0 = c_begin; c0 < c_end && __ballot(open0 || open1); c0 += 64) {
                 hlo[j0] = 0;
                 hlo[j1] = 0;
                 hhi[j0] = 0;
@@ -220,18 +232,60 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
                 if (cross0) open0 = false;
                 if (cross1) open1 = false;
             }
-            // entries: enter row Dr-1, delta(Dr-1, T) = -[row Dr-1 counted at T]
-            for (int w2 = 0; w2 < 2; ++w2) {
-                const int j = w2 ? j1 : j0, T = w2 ? T1 : T0;
-                if (j >= 1 && j < nch) {
-                    const int Dr = s - j * rc;
-                    const uint16_t e = U[v0 + Dr - 1];
-                    const int col = e & kIdx;
-                    const int ch = node * nch + j;
-                    bT[ch] = T;
-                    bD[ch] = -((e & kLoFlag) ? (col >= T) : (col < T));
+        };
+        // entries: enter row Dr-1, delta(Dr-1, T) = -[row Dr-1 counted at T]
+        auto entry = [&](int node, int v0, int j, int T) {
+            const int Dr = s - j * rc;
+            const uint16_t e = U[v0 + Dr - 1];
+            const int col = e & kIdx;
+            const int ch = node * nch + j;
+            bT[ch] = T;
+            bD[ch] = -((e & kLoFlag) ? (col >= T) : (col < T));
+        };
+        constexpr int kWaves = kThreads / 64;
+        const int nodes = K / s;
+        if (nodes >= kWaves || K < 64 * kWaves) {
+            // one wave per node: Y_j from the node's counts, then one sweep
+            for (int node = wave; node < nodes; node += kWaves) {
+                const int v0 = node * s;
+                int Y0, Y1, c0_, c1_;
+                range_counts(v0, 0, s, Y0, Y1, c0_, c1_);
+                int T0 = 0, T1 = 0;
+                find_T(v0, 0, s, Y0, Y1, 0, 0, j0 >= 1 && j0 < nch && Y0 > 0, j1 < nch && Y1 > 0, T0, T1);
+                if (j0 >= 1 && j0 < nch) entry(node, v0, j0, T0);
+                if (j1 < nch) entry(node, v0, j1, T1);
+            }
+        } else {
+            // W = kWaves / nodes waves per node, each on s / W consecutive
+            // columns: every wave counts its segment, the segments' counts
+            // give Y_j and each segment's starting Z_j, and only the wave in
+            // whose segment Z_j reaches Y_j sweeps for T_j.
+            const int W = kWaves / nodes, node = wave / W, seg = wave % W, seg_len = s / W;
+            const int v0 = node * s, cb = seg * seg_len;
+            int lp0, lp1, cnt0, cnt1;
+            range_counts(v0, cb, cb + seg_len, lp0, lp1, cnt0, cnt1);
+            hlo[j0] = cnt0;
+            hlo[j1] = cnt1;
+            hhi[j0] = lp0;
+            hhi[j1] = lp1;
+            __syncthreads();
+            int Y0 = 0, Y1 = 0, Zs0 = 0, Zs1 = 0;
+            for (int w2 = 0; w2 < W; ++w2) {
+                const int *other = hist + (node * W + w2) * 2 * kMaxChunks;
+                Y0 += other[kMaxChunks + j0];
+                Y1 += other[kMaxChunks + j1];
+                if (w2 < seg) {
+                    Zs0 += other[j0];
+                    Zs1 += other[j1];
                 }
             }
+            __syncthreads();   // every wave has read the counts before they are reused
+            const bool mine0 = j0 >= 1 && j0 < nch && Y0 > 0 && Zs0 < Y0 && Y0 <= Zs0 + cnt0;
+            const bool mine1 = j1 < nch && Y1 > 0 && Zs1 < Y1 && Y1 <= Zs1 + cnt1;
+            int T0 = 0, T1 = 0;
+            find_T(v0, cb, cb + seg_len, Y0, Y1, Zs0, Zs1, mine0, mine1, T0, T1);
+            if (j0 >= 1 && j0 < nch && (mine0 || (Y0 == 0 && seg == 0))) entry(node, v0, j0, T0);
+            if (j1 < nch && (mine1 || (Y1 == 0 && seg == 0))) entry(node, v0, j1, T1);
         }
     }
     __syncthreads();
