@@ -293,8 +293,11 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     // it reads kept in registers: cl = CI[dc-1], cc = CI[dc], ud = U[dr],
     // ur = U[dr-1].  A step right only needs the new CI[dc] and a step down
     // only the new U[dr-1], so every step is one LDS read from a selected
-    // address and no lane of the wave branches on the direction.
-    auto f_row = [](uint16_t e, int r) { return (e & kLoFlag) ? ((e & kIdx) >= r) : ((e & kIdx) < r); };
+    // address, and the step is written with selects so that no lane of the
+    // wave branches on the direction.
+    // counted(e, r) = lo ? idx >= r : idx < r for an entry e = lo << 15 | idx:
+    // bit 15 of e - r, for idx in [0, 32768) and r in [0, 32768].
+    auto counted = [](uint32_t e, int r) { return (int)(((e - (uint32_t)r) >> 15) & 1u); };
     for (int ch = threadIdx.x; ch < total; ch += kThreads) {
         const int v0 = (ch / nch) * s, j = ch % nch;
         const int r_hi = s - j * rc, r_lo = r_hi - rc;
@@ -308,27 +311,23 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
             delta = bD[ch];
         }
         // entries outside the node (dc = 0 or s, dr = 0) are read clamped and never used
-        uint16_t cl = CI[v0 + max(dc - 1, 0)], cc = CI[v0 + min(dc, s - 1)];
-        uint16_t ud = U[v0 + dr], ur = U[v0 + max(dr - 1, 0)];
+        uint32_t cl = CI[v0 + max(dc - 1, 0)], cc = CI[v0 + min(dc, s - 1)];
+        uint32_t ud = U[v0 + dr], ur = U[v0 + max(dr - 1, 0)];
         while (dr >= r_lo) {
-            if (dc > 0) {
-                const int d_left = delta - f_row(cl, dr);
-                const int d_down = delta + f_row(ud, dc);
-                if (d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
-            }
+            const int d_left = delta - counted(cl, dr);
+            const int d_down = delta + counted(ud, dc);
+            if (dc > 0 && d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
             // delta(dr, s) >= 0 always; the dc < s test only bounds the walk
             const bool right = delta < 0 && dc < s;
-            if (right) {
-                delta += f_row(cc, dr);
-                ++dc;
-                cl = cc;
-            } else {
-                if (dr > 0) delta -= f_row(ur, dc);
-                --dr;
-                ud = ur;
-            }
-            const uint16_t nx = right ? CI[v0 + min(dc, s - 1)] : U[v0 + max(dr - 1, 0)];
-            if (right) cc = nx; else ur = nx;
+            const int d_right = counted(cc, dr), d_up = dr > 0 ? counted(ur, dc) : 0;
+            delta = right ? delta + d_right : delta - d_up;
+            dc += right ? 1 : 0;
+            dr -= right ? 0 : 1;
+            cl = right ? cc : cl;
+            ud = right ? ud : ur;
+            const uint32_t nx = right ? CI[v0 + min(dc, s - 1)] : U[v0 + max(dr - 1, 0)];
+            cc = right ? nx : cc;
+            ur = right ? ur : nx;
         }
     }
     __syncthreads();
