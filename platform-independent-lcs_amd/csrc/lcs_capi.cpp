@@ -307,7 +307,13 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     if (!group_w) return LCS_EINVAL;
     if (!tm) {
         tm = default_tile_m(ctx, m);
-        if (strips_req != 1 && !d_h_in && !d_v_in && tm < 4 && default_strips(ctx, m, n, 4) > 1) tm = 4;
+        if (strips_req != 1 && !d_h_in && !d_v_in && tm < 4 && default_strips(ctx, m, n, 4) > 1) {
+            // a thin pair cut into strips: 8 rows per lane when at most 32
+            // strips of them fill every SIMD (fewer merge levels than with 4
+            // rows per lane would cost), else 4
+            const int s8 = default_strips(ctx, m, n, 8);
+            tm = (s8 <= 32 && (long long)s8 * bands_of(m, n, 8) >= 4LL * ctx->num_cus) ? 8 : 4;
+        }
     }
     int strips = strips_req ? strips_req : ((d_h_in || d_v_in) ? 1 : default_strips(ctx, m, n, tm));
     if (strips > 1 && (d_h_in || d_v_in)) return LCS_EINVAL;  // fixups use explicit strands: one strip
