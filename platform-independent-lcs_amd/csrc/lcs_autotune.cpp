@@ -1,6 +1,6 @@
 // The autotuner: times candidate schedules on synthetic pairs of a shape.
 // Its measurements are the training data of the cost model and the
-// exhaustive baseline that judges the model's choices.  Schedules only
+// exhaustive baseline that scores the model's choices.  Schedules only
 // change speed, so every candidate computes the same strands.
 #include <hip/hip_runtime.h>
 

@@ -11,7 +11,7 @@ own HIP kernels) to predict log2 cost per candidate from
     (log2 m, log2 n, log2 alphabet, mutation rate)        (scaled, see features)
 
 and the cheapest prediction is the schedule.  Exhaustive autotuning of the
-shape is the fallback and the judge of a choice (its regret: chosen cost /
+shape is the fallback and the yardstick of a choice (its regret: chosen cost /
 best cost).  Schedules change speed only; every candidate computes the same
 strands.
 

@@ -2,7 +2,7 @@
 
 The model is trained on the GPU (lcs_mlp_train) on the autotuner's timing
 table recorded on this device (committed under lcs_amd/data).  Held out by
-k-fold, its choice is judged by the exhaustive autotune's own measurements
+k-fold, its choice is scored by the exhaustive autotune's own measurements
 (regret = chosen cost / best cost) and compared with the built-in heuristic.
 It must pick schedules at least as good as the heuristic on average, and it
 decides in well under a millisecond where the exhaustive autotune spends
