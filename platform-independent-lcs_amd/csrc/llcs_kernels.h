@@ -42,13 +42,12 @@ struct LlcsArgs {
     int32_t pad;
 };
 
-constexpr int kLdsSymRing = 2048, kLdsInRing = 128, kLdsOutRing = 64;
+constexpr int kLdsSymRing = 2048, kLdsInRing = 128;
 // LDS: two buffers of 96 staged addresses, one all-zero row (padding
-// symbols), sigma rows, the symbol / carry-in / carry-out rings, 16 control
-// words
+// symbols), sigma rows, the symbol and carry-in rings, 16 control words
 inline size_t llcs_lds_bytes(int sigma)
 {
-    return (size_t)(2 * 96 + kWave + sigma * kWave + kLdsSymRing + kLdsInRing + kLdsOutRing + 16) * 4;
+    return (size_t)(2 * 96 + kWave + sigma * kWave + kLdsSymRing + kLdsInRing + 16) * 4;
 }
 
 }  // namespace lcsl

@@ -26,9 +26,10 @@
 //    lane's mask read is two dependent ds_reads with immediate offsets and no
 //    address arithmetic.
 //  * Band hand-off: lane 63's carry-out bits, 32 per chunk, go to the band
-//    above as one tagged 64-bit word (tag << 32 | bits) stored with an
-//    agent-scope relaxed atomic; the band above polls for its tag, bounded in
-//    time, with the abort flag of the comb kernels.  A carry-out bit is
+//    above as one tagged 64-bit word (tag << 32 | bits) that the comb wave
+//    stores with an agent-scope relaxed atomic; the fetch wave of the band
+//    above polls for its tag, bounded in time, with the abort flag of the
+//    comb kernels.  A carry-out bit is
 //    collected one step late, which aligns producer word c with symbols
 //    32(c-2) .. 32(c-2)+31: consumer chunk q reads word q+2.
 //  * Bands are handed out by an atomic ticket in order, so every wait is on a
@@ -53,17 +54,15 @@ constexpr int kStageWords = 96;              // one staging buffer (two are used
 constexpr int kZeroRowWords = kWave;         // row -1 of the table, all zero
 constexpr unsigned kNoTag = 0xffffffffu;
 constexpr int kSymRing = kLdsSymRing;        // LDS rings (powers of two): column symbols,
-constexpr int kInRing = kLdsInRing;          // carry words of the band below,
-constexpr int kOutRing = kLdsOutRing;        // this band's carry words
+constexpr int kInRing = kLdsInRing;          // carry words of the band below
 constexpr int kSymBatch = 1024;              // symbols per fetch round
 // LDS control words of a workgroup
-constexpr int kTask = 0, kGo = 1, kStop = 2, kInAvail = 3, kInUsed = 4, kSymAvail = 5, kSymUsed = 6, kOutAvail = 7,
-              kOutUsed = 8;
+constexpr int kTask = 0, kGo = 1, kStop = 2, kInAvail = 3, kInUsed = 4, kSymAvail = 5, kSymUsed = 6;
 
 // The workgroup's LDS: staging buffers, the zero row and the match-mask
-// table (comb wave), the three rings and the control words (shared).
+// table (comb wave), the two rings and the control words (shared).
 struct Lds {
-    lds_u32 *stage, *tab, *sym, *in, *out;
+    lds_u32 *stage, *tab, *sym, *in;
     lds_vu32 *ctl;
 };
 
@@ -182,12 +181,13 @@ __device__ __forceinline__ int32_t code_of(int32_t s, bool inside, int sigma, bo
     return s;
 }
 
-// Comb wave of a band: LDS only in its chunk loop.  Column symbols come from
-// the LDS symbol ring, the band below's carry words from the LDS in-ring, and
-// its own carry words go to the LDS out-ring; the fetch wave moves all three
-// between LDS and global memory.  (On gfx950 one counter, vmcnt, covers
-// loads and stores, so a global store in this loop would make the next
-// load's wait also wait for the store's device-scope write-through.)
+// Comb wave of a band.  Column symbols come from the LDS symbol ring and the
+// band below's carry words from the LDS in-ring, both filled by the fetch
+// wave; its own carry words go straight to the global ring, one store by
+// lane 0 per chunk.  The loop issues no global load, so nothing ever waits
+// for those stores (on gfx950 vmcnt counts loads and stores alike); staging
+// them through an LDS out-ring and the fetch wave instead cost 3 ms at
+// m = n = 2^20, in the chunk's own LDS traffic and in the longer hand-off.
 __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int lane, const Lds &L)
 {
     const int sigma = A.sigma;
@@ -212,7 +212,9 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
 
     const int32_t ncols = D.ncols, nchunks = D.nchunks;
     const bool has_in = g > 0, has_out = g + 1 < D.nbands;
-    uint32_t in_avail = 0, sym_avail = 0, out_used = 0;
+    gword *rout = (gword *)(D.ring + (size_t)((g + 1) & 1) * D.nchunks);
+    const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
+    uint32_t in_avail = 0, sym_avail = 0;
     bool ok = true;
 
     // Chunk c covers steps [32c, 32c+32); lane l takes column symbol k - l at
@@ -315,13 +317,7 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
         if (!ring_word(q + 1, win_next, use_next)) return false;
         quarter_steps<3>(V, Mc, win, wout, carry);
         read_quarter(Mn, 3);
-        if (has_out) {
-            if (q >= kOutRing && !wait_ctl(A, ctl, kOutUsed, (uint32_t)(q - kOutRing), out_used)) return false;
-            if (lane == 0) {
-                L.out[q & (kOutRing - 1)] = wout;
-                ctl[kOutAvail] = (uint32_t)q + 1;   // after the word: one wave's LDS writes land in order
-            }
-        }
+        if (has_out && lane == 0) st_relaxed(rout + q, tag_out | wout);
         return true;
     };
     int q = 0;
@@ -340,33 +336,28 @@ __device__ bool band_compute(const LlcsArgs &A, const LlcsDesc &D, int g, int la
     return true;
 }
 
-// Fetch wave of a band: all of the band's global memory traffic after its
-// table is built.  Each round it
+// Fetch wave of a band: the band's global loads after its table is built.
+// Each round it
 //  * loads the next column symbols (up to kSymBatch, 16 per lane) into the
 //    LDS symbol ring, as far as the comb wave's progress frees slots;
 //  * loads the band below's next 64 carry words from the global ring and
 //    copies the leading run that carries this band's tag to the LDS in-ring
-//    (one load brings every word published so far);
-//  * stores the comb wave's new carry words from the LDS out-ring to the
-//    global ring, tagged for the band above.
-// Loads are issued before any result is awaited, so a round costs about one
-// memory round trip.  Every wait is bounded in time.
+//    (one load brings every word published so far).
+// Both loads are issued before either result is awaited, so a round costs
+// about one memory round trip.  Every wait is bounded in time.
 __device__ void band_fetch(const LlcsArgs &A, const LlcsDesc &D, int g, int lane, const Lds &L)
 {
     lds_vu32 *ctl = L.ctl;
     const int32_t ncols = D.ncols, nchunks = D.nchunks;
-    const bool need_in = g > 0 && nchunks > 2, need_out = g + 1 < D.nbands;
+    const bool need_in = g > 0 && nchunks > 2;
     gcword *rin = (gcword *)(D.ring + (size_t)(g & 1) * nchunks);
-    gword *rout = (gword *)(D.ring + (size_t)((g + 1) & 1) * nchunks);
     const uint32_t tag_in = (uint32_t)g;
-    const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
     gsym *cols = (gsym *)D.cols;
-    int32_t next_s = 0, next_w = 2, out_done = 0;
+    int32_t next_s = 0, next_w = 2;
     unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
     for (;;) {
         const bool done_s = next_s >= ncols, done_w = !need_in || next_w >= nchunks;
-        const bool done_o = !need_out || out_done >= nchunks;
-        if (done_s && done_w && done_o) return;
+        if (done_s && done_w) return;
         if (__builtin_amdgcn_readfirstlane(ctl[kStop])) return;
         bool progress = false;
 
@@ -388,17 +379,6 @@ __device__ void band_fetch(const LlcsArgs &A, const LlcsDesc &D, int g, int lane
             const int32_t used = (int32_t)__builtin_amdgcn_readfirstlane(ctl[kInUsed]);
             want = min(min(used + kInRing - next_w, kWave), nchunks - next_w);
             if (want > 0 && lane < want) e = ld_relaxed(rin + next_w + lane);
-        }
-        // carry words out
-        if (!done_o) {
-            const int32_t avail = (int32_t)__builtin_amdgcn_readfirstlane(ctl[kOutAvail]);
-            const int32_t k = min(avail - out_done, kWave);
-            if (k > 0) {
-                if (lane < k) st_relaxed(rout + out_done + lane, tag_out | L.out[(out_done + lane) & (kOutRing - 1)]);
-                out_done += k;
-                if (lane == 0) ctl[kOutUsed] = (uint32_t)out_done;   // after the reads of those slots
-                progress = true;
-            }
         }
         // land the loads
         if (scnt > 0) {
@@ -445,8 +425,7 @@ __global__ __launch_bounds__(2 * kWave) void llcs_chain_kernel(LlcsArgs A)
     L.tab = L.stage + 2 * kStageWords + kZeroRowWords;
     L.sym = L.tab + A.sigma * kWave;
     L.in = L.sym + kSymRing;
-    L.out = L.in + kInRing;
-    L.ctl = (lds_vu32 *)(L.out + kOutRing);
+    L.ctl = (lds_vu32 *)(L.in + kInRing);
     lds_vu32 *ctl = L.ctl;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
     for (;;) {
@@ -459,8 +438,6 @@ __global__ __launch_bounds__(2 * kWave) void llcs_chain_kernel(LlcsArgs A)
             ctl[kInUsed] = 2u;
             ctl[kSymAvail] = 0u;
             ctl[kSymUsed] = 0u;
-            ctl[kOutAvail] = 0u;
-            ctl[kOutUsed] = 0u;
         }
         __syncthreads();
         if (!__builtin_amdgcn_readfirstlane(ctl[kGo])) break;
