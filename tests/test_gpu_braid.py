@@ -22,7 +22,9 @@ def test_core_random_permutations(ctx):
             assert np.array_equal(got, exp), (k, glue)
 
 
-@pytest.mark.parametrize("k", [1, 2, 31, 32, 33, 100, 1000, 4097, 16384])
+# 16384 is the largest core kept wholly in LDS; 20000 and 32768 take the
+# global-memory form of the same kernel (16-bit entries up to K = 32768)
+@pytest.mark.parametrize("k", [1, 2, 31, 32, 33, 100, 1000, 4097, 16384, 16385, 20000, 32768])
 def test_core_sizes(ctx, k):
     rng = np.random.default_rng(k)
     p = rng.permutation(k).astype(np.int32)
