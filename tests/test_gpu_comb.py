@@ -136,6 +136,18 @@ def test_thin_pair_default_schedule(ctx):
     assert np.array_equal(h, ho) and np.array_equal(v, vo)
 
 
+def test_thin_pair_eight_row_strips(ctx):
+    # m = 2^14, n = 2^19: the default schedule combs 32 strips of 32 bands of
+    # 8 rows per lane and merges them with five steady-ant levels; the result
+    # must equal the pair combed as one strip (itself pinned to the oracle by
+    # the tests above)
+    a = lcs_amd.gen_zipf(11, 1 << 14, 26, 1.1)
+    b = lcs_amd.gen_planted(12, a, 1 << 19, 26, 1.1, 8, 0.05)
+    h, v, llcs = ctx.comb(a, b)
+    h1, v1, llcs1 = ctx.comb(a, b, lcs_amd.schedule(strips=1))
+    assert np.array_equal(h, h1) and np.array_equal(v, v1) and llcs == llcs1
+
+
 LDS = 2  # LCS_VARIANT_CHAIN_LDS
 
 
