@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "braid_kernels.h"
@@ -472,20 +473,32 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 
 // ---------------------------------------------------------------- slicing
 // Device-wide stable compaction in three kernels: per-block counts, a scan
-// of the counts in one block, per-block scatter.
+// of the counts in one block, per-block scatter.  The products of one merge
+// level go through each kernel in one launch: blockIdx.y picks the slice
+// (two per product, p's rows at even y and q's columns at odd y).
 constexpr int kScanBlock = 256, kScanPer = 8, kScanTile = kScanBlock * kScanPer;
+constexpr int kMulBatch = 32;   // products per slicing / assembly launch
 
 struct SliceSpec {
-    // selects element i of `src` (size n) when lo <= key(i) < hi, where
-    // key(i) = src[i]; records out_map[j] = i and out_val[key] or out_val[j]
+    // selects element i of `src` (size n) when lo <= src[i] < hi; per-block
+    // counts (then their exclusive scan) in `counts`; records, for slice j of
+    // the selected elements (j < cap):
+    //   even slice (vertical slice of p, rows in order):   map[j] = i, val[j] = src[i] - lo
+    //   odd slice (horizontal slice of q, cols in order):  map[j] = i (a column), val[src[i] - lo] = j
     const int32_t *src;
-    int32_t n, lo, hi;
+    int32_t *counts, *map, *val;
+    int32_t n, lo, hi, cap;
+};
+struct SliceBatch {
+    SliceSpec s[2 * kMulBatch];
 };
 
 __device__ __forceinline__ bool sel(const SliceSpec &S, int i) { const int v = S.src[i]; return v >= S.lo && v < S.hi; }
 
-__global__ void slice_count_kernel(SliceSpec S, int32_t *block_counts)
+__global__ void slice_count_kernel(SliceBatch B)
 {
+    const SliceSpec &S = B.s[blockIdx.y];
+    if ((int64_t)blockIdx.x * kScanTile >= S.n) return;   // past this slice's blocks (whole block)
     __shared__ int32_t red[kScanBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int c = 0;
@@ -496,12 +509,15 @@ __global__ void slice_count_kernel(SliceSpec S, int32_t *block_counts)
         if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) block_counts[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) S.counts[blockIdx.x] = red[0];
 }
 
-__global__ void scan_counts_kernel(int32_t *counts, int nblocks)
+__global__ void scan_counts_kernel(SliceBatch B)
 {
-    // exclusive scan of counts[0..nblocks) in place, one block of 1024
+    // exclusive scan of slice blockIdx.x's block counts in place, one block of 1024
+    const SliceSpec &S = B.s[blockIdx.x];
+    int32_t *counts = S.counts;
+    const int nblocks = (S.n + kScanTile - 1) / kScanTile;
     __shared__ int32_t s[1024];
     const int t = threadIdx.x;
     const int per = (nblocks + 1023) / 1024;
@@ -520,11 +536,11 @@ __global__ void scan_counts_kernel(int32_t *counts, int nblocks)
     for (int i = lo; i < hi; ++i) { const int c = counts[i]; counts[i] = acc; acc += c; }
 }
 
-// mode 0 (vertical slice of p, rows in order):   map[j] = i, val[j] = src[i] - lo
-// mode 1 (horizontal slice of q, cols in order): map[j] = i (a column), val[src[i] - lo] = j
-__global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, int32_t *map, int32_t *val, int mode,
-                                     int cap)
+__global__ void slice_scatter_kernel(SliceBatch B)
 {
+    const SliceSpec &S = B.s[blockIdx.y];
+    if ((int64_t)blockIdx.x * kScanTile >= S.n) return;   // past this slice's blocks (whole block)
+    const bool rows = (blockIdx.y & 1) == 0;
     __shared__ int32_t s[kScanBlock];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     int c = 0;
@@ -537,28 +553,42 @@ __global__ void slice_scatter_kernel(SliceSpec S, const int32_t *block_offsets, 
         s[threadIdx.x] += v;
         __syncthreads();
     }
-    int j = block_offsets[blockIdx.x] + s[threadIdx.x] - c;
+    int j = S.counts[blockIdx.x] + s[threadIdx.x] - c;
     for (int q = 0; q < kScanPer; ++q) {
         const int i = base + q;
-        if (i < S.n && sel(S, i) && j < cap) {
-            map[j] = i;
-            if (mode == 0) val[j] = S.src[i] - S.lo;
-            else val[S.src[i] - S.lo] = j;
+        if (i < S.n && sel(S, i) && j < S.cap) {
+            S.map[j] = i;
+            if (rows) S.val[j] = S.src[i] - S.lo;
+            else S.val[S.src[i] - S.lo] = j;
             ++j;
         }
     }
 }
 
-__global__ void invert_kernel(const int32_t *r2c, int32_t n, int32_t *c2r)
+struct InvertBatch {
+    struct {
+        const int32_t *r2c;
+        int32_t *c2r;
+        int32_t n;
+    } s[kMulBatch];
+};
+
+__global__ void invert_kernel(InvertBatch B)
 {
+    const auto &S = B.s[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (uint32_t)r2c[i] < (uint32_t)n) c2r[r2c[i]] = i;
+    if (i < S.n && (uint32_t)S.r2c[i] < (uint32_t)S.n) S.c2r[S.r2c[i]] = i;
 }
+
+struct AssembleBatch {
+    AssembleArgs a[kMulBatch];
+};
 
 // Writes the product (row -> col) from the core result and the untouched
 // points of p and q (braid_multiplication.hpp:353-401).
-__global__ void assemble_kernel(AssembleArgs A)
+__global__ void assemble_kernel(AssembleBatch B)
 {
+    const AssembleArgs &A = B.a[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = A.k, ps = A.p_size, qs = A.q_size;
     // the range tests only matter after an aborted comb fed undefined input
@@ -632,61 +662,78 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
 }
 
 // staggered_multiply for `count` independent products on device pointers.
-// Each product's slicing and assembly are a few small launches; the cores of
-// all products go to one launch (one workgroup each), so the products of a
-// merge-tree level run side by side.  specs[i].ws holds
+// Their slicing, cores and assembly each go to one launch per kind (in
+// groups of kMulBatch products for the slicing and assembly), so the
+// products of a merge-tree level run side by side.  specs[i].ws holds
 // lcsb_multiply_ws_words(p_size, q_size, k) words.
 int lcsb_multiply_many_device(const MulSpec *specs, int count, hipStream_t stream)
 {
     std::vector<BraidCoreDesc> cores;
     cores.reserve((size_t)count);
-    for (int i = 0; i < count; ++i) {
-        const MulSpec &M = specs[i];
-        const int32_t p_size = M.p_size, q_size = M.q_size, k = M.k;
-        const int nmax = p_size > q_size ? p_size : q_size;
-        int32_t *q_c2r = M.ws;                // q_size
-        int32_t *map_row = q_c2r + q_size;    // k
-        int32_t *map_col = map_row + k;       // k
-        int32_t *Pred = map_col + k;          // k
-        int32_t *Qred = Pred + k;             // k
-        int32_t *core = Qred + k;             // k
-        int32_t *counts = core + k;           // nblocks
-        const int nblocks_max = (nmax + kScanTile - 1) / kScanTile + 1;
-        int32_t *core_ws = counts + nblocks_max;
-        if (k == 0) continue;
-        hipLaunchKernelGGL(invert_kernel, dim3((q_size + 255) / 256), dim3(256), 0, stream, M.q, q_size, q_c2r);
-        // vertical slice of p: rows whose column lies in the glued range
-        SliceSpec sp{M.p, p_size, M.row_glue ? 0 : p_size - k, M.row_glue ? k : p_size};
-        int nb = (p_size + kScanTile - 1) / kScanTile;
-        hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts);
-        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
-        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sp, counts, map_row, Pred, 0, k);
-        // horizontal slice of q: columns whose row lies in the glued range
-        SliceSpec sq{q_c2r, q_size, M.row_glue ? q_size - k : 0, M.row_glue ? q_size : k};
-        nb = (q_size + kScanTile - 1) / kScanTile;
-        hipLaunchKernelGGL(slice_count_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts);
-        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, stream, counts, nb);
-        hipLaunchKernelGGL(slice_scatter_kernel, dim3(nb), dim3(kScanBlock), 0, stream, sq, counts, map_col, Qred, 1, k);
-        BraidCoreDesc h{};
-        h.P = Pred;
-        h.Q = Qred;
-        h.R = core;
-        h.k = k;
-        h.L = lcsb_core_L(k);
-        h.scratch = core_ws;
-        cores.push_back(h);
+    auto nblocks = [](int32_t n) { return (n + kScanTile - 1) / kScanTile; };
+    for (int base = 0; base < count; base += kMulBatch) {
+        const int cnt = count - base < kMulBatch ? count - base : kMulBatch;
+        InvertBatch ib{};
+        SliceBatch sb{};
+        int ni = 0, inv_blocks = 0, slice_blocks = 0;
+        for (int i = 0; i < cnt; ++i) {
+            const MulSpec &M = specs[base + i];
+            const int32_t p_size = M.p_size, q_size = M.q_size, k = M.k;
+            if (k == 0) continue;
+            const int nmax = p_size > q_size ? p_size : q_size;
+            const int nblocks_max = nblocks(nmax) + 1;
+            int32_t *q_c2r = M.ws;                // q_size
+            int32_t *map_row = q_c2r + q_size;    // k
+            int32_t *map_col = map_row + k;       // k
+            int32_t *Pred = map_col + k;          // k
+            int32_t *Qred = Pred + k;             // k
+            int32_t *core = Qred + k;             // k
+            int32_t *counts_p = core + k;         // nblocks_max
+            int32_t *counts_q = counts_p + nblocks_max;
+            int32_t *core_ws = counts_q + nblocks_max;
+            ib.s[ni].r2c = M.q;
+            ib.s[ni].c2r = q_c2r;
+            ib.s[ni].n = q_size;
+            // vertical slice of p: rows whose column lies in the glued range
+            sb.s[2 * ni] = SliceSpec{M.p, counts_p, map_row, Pred, p_size, M.row_glue ? 0 : p_size - k,
+                                     M.row_glue ? k : p_size, k};
+            // horizontal slice of q: columns whose row lies in the glued range
+            sb.s[2 * ni + 1] = SliceSpec{q_c2r, counts_q, map_col, Qred, q_size, M.row_glue ? q_size - k : 0,
+                                         M.row_glue ? q_size : k, k};
+            ++ni;
+            inv_blocks = std::max(inv_blocks, (q_size + 255) / 256);
+            slice_blocks = std::max(slice_blocks, std::max(nblocks(p_size), nblocks(q_size)));
+            BraidCoreDesc h{};
+            h.P = Pred;
+            h.Q = Qred;
+            h.R = core;
+            h.k = k;
+            h.L = lcsb_core_L(k);
+            h.scratch = core_ws;
+            cores.push_back(h);
+        }
+        if (ni == 0) continue;
+        hipLaunchKernelGGL(invert_kernel, dim3(inv_blocks, ni), dim3(256), 0, stream, ib);
+        hipLaunchKernelGGL(slice_count_kernel, dim3(slice_blocks, 2 * ni), dim3(kScanBlock), 0, stream, sb);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(2 * ni), dim3(1024), 0, stream, sb);
+        hipLaunchKernelGGL(slice_scatter_kernel, dim3(slice_blocks, 2 * ni), dim3(kScanBlock), 0, stream, sb);
     }
     if (!cores.empty()) {
         const int rc = lcsb_launch_core(cores.data(), (int)cores.size(), stream);
         if (rc) return rc;
     }
-    for (int i = 0; i < count; ++i) {
-        const MulSpec &M = specs[i];
-        const int nmax = M.p_size > M.q_size ? M.p_size : M.q_size;
-        int32_t *map_row = M.ws + M.q_size, *map_col = map_row + M.k, *core = map_col + 3 * (size_t)M.k;
-        AssembleArgs A{M.p, M.q, M.out, map_row, map_col, core, M.p_size, M.q_size, M.k, M.row_glue};
-        const int n = nmax > M.k ? nmax : M.k;
-        hipLaunchKernelGGL(assemble_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, A);
+    for (int base = 0; base < count; base += kMulBatch) {
+        const int cnt = count - base < kMulBatch ? count - base : kMulBatch;
+        AssembleBatch ab{};
+        int blocks = 1;
+        for (int i = 0; i < cnt; ++i) {
+            const MulSpec &M = specs[base + i];
+            int32_t *map_row = M.ws + M.q_size, *map_col = map_row + M.k, *core = map_col + 3 * (size_t)M.k;
+            ab.a[i] = AssembleArgs{M.p, M.q, M.out, map_row, map_col, core, M.p_size, M.q_size, M.k, M.row_glue};
+            const int n = std::max(std::max(M.p_size, M.q_size), M.k);
+            blocks = std::max(blocks, (n + 255) / 256);
+        }
+        hipLaunchKernelGGL(assemble_kernel, dim3(blocks, cnt), dim3(256), 0, stream, ab);
     }
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
@@ -703,5 +750,5 @@ size_t lcsb_multiply_ws_words(int32_t p_size, int32_t q_size, int32_t k)
 {
     const int nmax = p_size > q_size ? p_size : q_size;
     const size_t nblocks_max = (size_t)(nmax + kScanTile - 1) / kScanTile + 1;
-    return (size_t)q_size + 5 * (size_t)k + nblocks_max + lcsb_core_scratch_words(k > 0 ? k : 1);
+    return (size_t)q_size + 5 * (size_t)k + 2 * nblocks_max + lcsb_core_scratch_words(k > 0 ? k : 1);
 }
