@@ -118,7 +118,9 @@ def test_repeatable(ctx):
     assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
 
 
-@pytest.mark.parametrize("strips", [2, 3, 8])
+# 80 strips: the first merge level has 40 products, more than one slicing
+# launch takes (32), and later levels carry an odd permutation over
+@pytest.mark.parametrize("strips", [2, 3, 8, 80])
 def test_column_strips_merged_by_steady_ant(ctx, strips):
     rng = np.random.default_rng(strips)
     a = rng.integers(0, 4, 3000).astype(np.int32)
