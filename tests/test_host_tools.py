@@ -1,5 +1,4 @@
 """Host-side logic of the tuning flow and the bench contract (no GPU)."""
-import json
 import os
 import subprocess
 import sys
