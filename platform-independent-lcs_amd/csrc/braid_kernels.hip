@@ -380,17 +380,71 @@ __global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
 // One depth of the top-down split with the keys in LDS: a stable partition
 // of every aligned segment of size s by bit `bit` (zeros first), the ranks
 // from wave ballots and one scan of the 64-key group counts.
-// par[pos] = old position (global).  K >= 64.
+// par[pos] = old position (global).  K >= 64.  A wave takes the groups
+// wave, wave + 16, ...; U of them at a time, with every load of the U groups
+// issued before the first ballot, so their LDS latencies overlap (the
+// compiler does not unroll a loop around a ballot by itself).
+template <int U>
+__device__ __forceinline__ void split_count_groups(const uint16_t *key, int bit, int32_t *gpre, int g0, int lane)
+{
+    uint16_t kv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kv[u] = key[(g0 + u * (kThreads / 64)) * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const unsigned long long m = __ballot((kv[u] >> bit) & 1);
+        if (lane == 0) gpre[g0 + u * (kThreads / 64)] = __popcll(m);
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t *key_new, int32_t *par, int s,
+                                                   int bit, const int32_t *gpre, int g0, int lane)
+{
+    const unsigned long long lt = (1ull << lane) - 1;
+    uint16_t kv[U];
+    int gp[U], gs0[U], gs1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int g = g0 + u * (kThreads / 64), x = g * 64 + lane;
+        kv[u] = key[x];
+        if (s >= 64) {
+            const int gs = (x & ~(s - 1)) >> 6;
+            gp[u] = gpre[g];
+            gs0[u] = gpre[gs];
+            gs1[u] = gpre[gs + (s >> 6)];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int g = g0 + u * (kThreads / 64), x = g * 64 + lane;
+        const int b = (kv[u] >> bit) & 1;
+        const unsigned long long m = __ballot(b);
+        const int seg = x & ~(s - 1);
+        int before, ones;
+        if (s >= 64) {
+            before = gp[u] - gs0[u] + __popcll(m & lt);
+            ones = gs1[u] - gs0[u];
+        } else {
+            const unsigned long long segmask = ((1ull << s) - 1) << (lane & ~(s - 1));
+            before = __popcll(m & lt & segmask);
+            ones = __popcll(m & segmask);
+        }
+        const int pos = b ? seg + (s - ones) + before : seg + (x - seg - before);
+        key_new[pos] = kv[u];
+        par[pos] = x;
+    }
+}
+
 __device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t *par, int K, int s, int bit,
                                 int32_t *gpre)
 {
+    constexpr int kW = kThreads / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ngroups = K >> 6;
-    const unsigned long long lt = (1ull << lane) - 1;
-    for (int g = wave; g < ngroups; g += kThreads / 64) {
-        const unsigned long long m = __ballot((key[g * 64 + lane] >> bit) & 1);
-        if (lane == 0) gpre[g] = __popcll(m);
-    }
+    int g = wave;
+    for (; g + 3 * kW < ngroups; g += 4 * kW) split_count_groups<4>(key, bit, gpre, g, lane);
+    for (; g < ngroups; g += kW) split_count_groups<1>(key, bit, gpre, g, lane);
     __syncthreads();
     if (wave == 0) {  // exclusive scan of the group counts, gpre[ngroups] = total
         int carry = 0;
@@ -403,26 +457,9 @@ __device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t 
         if (lane == 0) gpre[ngroups] = carry;
     }
     __syncthreads();
-    for (int g = wave; g < ngroups; g += kThreads / 64) {
-        const int x = g * 64 + lane;
-        const uint16_t kv = key[x];
-        const int b = (kv >> bit) & 1;
-        const unsigned long long m = __ballot(b);
-        const int seg = x & ~(s - 1);
-        int before, ones;
-        if (s >= 64) {
-            const int gs = seg >> 6;
-            before = gpre[g] - gpre[gs] + __popcll(m & lt);
-            ones = gpre[gs + (s >> 6)] - gpre[gs];
-        } else {
-            const unsigned long long segmask = ((1ull << s) - 1) << (lane & ~(s - 1));
-            before = __popcll(m & lt & segmask);
-            ones = __popcll(m & segmask);
-        }
-        const int pos = b ? seg + (s - ones) + before : seg + (x - seg - before);
-        key_new[pos] = kv;
-        par[pos] = x;
-    }
+    g = wave;
+    for (; g + 3 * kW < ngroups; g += 4 * kW) split_place_groups<4>(key, key_new, par, s, bit, gpre, g, lane);
+    for (; g < ngroups; g += kW) split_place_groups<1>(key, key_new, par, s, bit, gpre, g, lane);
     __syncthreads();
 }
 
