@@ -89,6 +89,16 @@ int lcs_debug_band_trace(lcs_ctx *ctx, int32_t max_bands, uint64_t *host_out);
 int lcs_comb(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
              const lcs_schedule *sched, int32_t *h_out, int32_t *v_out, int32_t *llcs_out);
 
+/* Replaces SemiLocalLcsImpl::operator() (src/algorithm_registry.cpp:43-75):
+ * the procedure followed by PermutationMatrix::FromStrands
+ * (src/permutation_matrix.hpp:88-115), which the reference runs on the host
+ * inside its timed region (:70-71).  Host pointers, synchronous.  Combs on
+ * the GPU, builds the permutation there and copies out row_to_col and
+ * col_to_row (m+n entries each, the reference's layout: strand h[l] ends in
+ * column n+l, v[j] in column j); llcs_out (nullable) as for lcs_comb. */
+int lcs_comb_permutation(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+                         const lcs_schedule *sched, int32_t *row_to_col, int32_t *col_to_row, int32_t *llcs_out);
+
 /* Same computation on device pointers, enqueued on `stream` (a hipStream_t;
  * NULL = the context's stream).  Returns after enqueueing; the caller
  * synchronises the stream.  Workspace comes from the context. */

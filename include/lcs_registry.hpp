@@ -67,6 +67,7 @@ public:
     bool empty() const { return size_ == 0; }
 
 private:
+    friend struct SemiLocalLcsImpl;   // fills both arrays from the GPU (lcs_comb_permutation)
     std::vector<int> row_to_col_, col_to_row_;
     int size_ = 0;
 };
@@ -133,6 +134,9 @@ typedef void LcsProcedure(LcsProblemContext &ctx);
 struct SemiLocalLcsImpl {
     bool ok = false;
     bool score_only = false;   // the procedure fills length_of_lcs only (no strands)
+    // the procedure is the GPU comb: operator() builds the permutation on the
+    // GPU (lcs_comb_permutation) instead of FromStrands on the host
+    bool device_permutation = false;
     lcs_ctx *queue = nullptr;
     LcsFunction *func = nullptr;
     LcsProcedure *proc = nullptr;

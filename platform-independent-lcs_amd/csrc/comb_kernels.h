@@ -47,6 +47,9 @@ int lcsk_launch_identity(int32_t *h_out, int32_t m, int32_t *v_out, int32_t n, h
 // each, written at perm + perm_off[p]).
 int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, int32_t *perm,
                              const int64_t *d_perm_off, hipStream_t stream);
+// row_to_col (size m+n) of one pair's permutation from its device strands.
+int lcsk_launch_r2c_from_strands(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c,
+                                 hipStream_t stream);
 // Strands from a permutation (row_to_col, size m+n): h_out[l] = row of col
 // n+l, v_out[j] = row of col j.  scratch holds m+n words.
 int lcsk_launch_strands_from_perm(const int32_t *r2c, int32_t m, int32_t n, int32_t *scratch, int32_t *h_out,

@@ -637,6 +637,19 @@ __global__ void from_strands_kernel(const PairDesc *pairs, int32_t *perm, const 
     }
 }
 
+// row_to_col of one pair's permutation from its device strands (FromStrands,
+// src/permutation_matrix.hpp:88-115): h[l] -> n+l, v[j] -> j.
+__global__ void r2c_from_strands_kernel(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t size = (uint32_t)(m + n);
+    if (i >= m + n) return;
+    // ids are always < m+n after a completed comb; the test keeps an aborted
+    // one (whose strands are undefined) from writing out of range
+    const uint32_t id = (uint32_t)(i < m ? h[i] : v[i - m]);
+    if (id < size) r2c[id] = i < m ? n + i : i - m;
+}
+
 __global__ void invert_perm_kernel(const int32_t *r2c, int32_t size, int32_t *c2r)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -709,6 +722,15 @@ int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, i
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(from_strands_kernel, dim3(blocks, npairs), dim3(256), 0, stream, d_pairs, perm, d_perm_off);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_r2c_from_strands(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c,
+                                 hipStream_t stream)
+{
+    const int size = m + n;
+    if (size == 0) return 0;
+    hipLaunchKernelGGL(r2c_from_strands_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, h, m, v, n, r2c);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -398,6 +398,7 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     ctx->in_b.release(ctx);
     ctx->out_h.release(ctx);
     ctx->out_v.release(ctx);
+    ctx->perm.release(ctx);
     if (ctx->ctrl) (void)hipFree(ctx->ctrl);
     if (ctx->band_trace) (void)hipFree(ctx->band_trace);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -456,6 +457,46 @@ int lcs_comb(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_
     if (llcs_out) {
         int32_t c = 0;
         for (int32_t j = 0; j < n; ++j) c += v_out[j] < m;
+        *llcs_out = c;
+    }
+    return LCS_OK;
+}
+
+int lcs_comb_permutation(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+                         const lcs_schedule *sched, int32_t *row_to_col, int32_t *col_to_row, int32_t *llcs_out)
+{
+    if (!ctx || m < 0 || n < 0) return LCS_EINVAL;
+    if ((m > 0 && !a) || (n > 0 && !b) || (m + n > 0 && (!row_to_col || !col_to_row))) return LCS_EINVAL;
+    if ((long long)m + n > 0x7fffffffLL) return LCS_ERANGE;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    int rc;
+    if ((rc = ctx->in_a.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
+    if ((rc = ctx->in_b.reserve(ctx, (size_t)std::max(n, 1) * 4))) return rc;
+    if ((rc = ctx->out_h.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
+    if ((rc = ctx->out_v.reserve(ctx, (size_t)std::max(n, 1) * 4))) return rc;
+    if ((rc = ctx->perm.reserve(ctx, (size_t)std::max(m + n, 1) * 4))) return rc;
+    int32_t *da = (int32_t *)ctx->in_a.ptr, *db = (int32_t *)ctx->in_b.ptr;
+    int32_t *dh = (int32_t *)ctx->out_h.ptr, *dv = (int32_t *)ctx->out_v.ptr, *dr2c = (int32_t *)ctx->perm.ptr;
+    hipStream_t s = ctx->stream;
+    if (m && hipMemcpyAsync(da, a, (size_t)m * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
+    if (n && hipMemcpyAsync(db, b, (size_t)n * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
+    if (hipEventRecord(ctx->ev0, s) != hipSuccess) return LCS_EIO;
+    if ((rc = enqueue_chain(ctx, da, m, db, n, sched, dh, dv, s))) return rc;
+    if (hipEventRecord(ctx->ev1, s) != hipSuccess) return LCS_EIO;
+    if (lcsk_launch_r2c_from_strands(dh, m, dv, n, dr2c, s)) return LCS_EIO;
+    // col_to_row is v then h (FromStrands: col j <- v[j], col n+l <- h[l])
+    if (n && hipMemcpyAsync(col_to_row, dv, (size_t)n * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
+    if (m && hipMemcpyAsync(col_to_row + n, dh, (size_t)m * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LCS_EIO;
+    if (m + n && hipMemcpyAsync(row_to_col, dr2c, (size_t)(m + n) * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LCS_EIO;
+    if (hipStreamSynchronize(s) != hipSuccess) return LCS_EIO;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_kernel_ms = ms;
+    if ((rc = check_abort(ctx))) return rc;
+    if (llcs_out) {
+        int32_t c = 0;
+        for (int32_t j = 0; j < n; ++j) c += col_to_row[j] < m;
         *llcs_out = c;
     }
     return LCS_OK;

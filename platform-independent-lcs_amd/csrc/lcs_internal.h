@@ -40,7 +40,7 @@ struct lcs_ctx {
     unsigned long long timeout_ticks = 500000000ull;  // 5 s at 100 MHz
     double last_kernel_ms = 0.0;
     bool pending_timing = false;
-    lcs::DevBuf ring, in_a, in_b, out_h, out_v, descs, strip_ws;
+    lcs::DevBuf ring, in_a, in_b, out_h, out_v, descs, strip_ws, perm;
     hipEvent_t upload_done = nullptr;    // last copy out of `pinned`
     void *pinned = nullptr;              // pinned staging for descriptor uploads
     size_t pinned_bytes = 0;

@@ -184,6 +184,17 @@ PermutationMatrix SemiLocalLcsImpl::operator()(const LcsInput &input)
                                                   ctx.v_strands_size);
         return std::move(ctx.matrix);
     }
+    if (proc && device_permutation && queue) {
+        // the same result as the procedure + FromStrands below, with the
+        // permutation built on the GPU
+        const int m = input.a_size, n = input.b_size;
+        if (m < 0 || n < 0) return PermutationMatrix();
+        PermutationMatrix pm(m + n);
+        if (lcs_comb_permutation(queue, input.a_data, m, input.b_data, n, &schedule, pm.row_to_col_.data(),
+                                 pm.col_to_row_.data(), nullptr) != LCS_OK)
+            return PermutationMatrix();
+        return pm;
+    }
     if (proc) {
         LcsProblemContext p;
         p.a_given = reinterpret_cast<const Word *>(input.a_data);
@@ -247,6 +258,7 @@ SemiLocalLcsImpl LcsAlgorithmRegistry::Get(const std::string &name, const std::s
         result.proc = reg2[name];
         result.schedule = schedules[name];
         result.score_only = score_only.count(name) > 0;
+        result.device_permutation = result.proc == Lcs_Semi_Hip;
         result.ok = true;  // as in the reference: a known name is ok, the queue may still be null
         return result;
     }

@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
     L.lcs_ctx_device_name.argtypes = [vp, ctypes.c_char_p, sz]
     L.lcs_ctx_info.argtypes = [vp, _i32p, ctypes.POINTER(ctypes.c_double)]
     L.lcs_comb.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
+    L.lcs_comb_permutation.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
     L.lcs_comb_device.argtypes = [vp, vp, i32, vp, i32, sched_p, vp, vp, vp]
     L.lcs_comb_fixup.argtypes = [vp, _i32p, i32, _i32p, i32, _i32p, _i32p, i32, i32, i32, i32]
     L.lcs_permutation_from_strands.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p]
@@ -194,6 +195,20 @@ class Context:
                             _p(h), _p(v), ctypes.byref(llcs))
         _check("lcs_comb", rc)
         return h[: a.size].copy(), v[: b.size].copy(), llcs.value
+
+    def comb_permutation(self, a, b, sched: Schedule | None = None):
+        """(row_to_col, col_to_row, llcs) of a against b, the permutation built
+        on the GPU (lcs_comb_permutation)."""
+        a, b = _i32(a), _i32(b)
+        size = a.size + b.size
+        r2c = np.empty(max(size, 1), np.int32)
+        c2r = np.empty(max(size, 1), np.int32)
+        llcs = ctypes.c_int32(-1)
+        rc = lib().lcs_comb_permutation(self._h, _p(a), a.size, _p(b), b.size,
+                                        ctypes.byref(sched) if sched is not None else None,
+                                        _p(r2c), _p(c2r), ctypes.byref(llcs))
+        _check("lcs_comb_permutation", rc)
+        return r2c[:size].copy(), c2r[:size].copy(), llcs.value
 
     def comb_fixup(self, a_prepared, b, h, v, i0, isize, j0, jsize):
         ap, b = _i32(a_prepared), _i32(b)

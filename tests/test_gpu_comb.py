@@ -192,3 +192,30 @@ def test_lds_groups_reference_sizes_and_strips(ctx):
 def test_unknown_variant_is_rejected(ctx):
     with pytest.raises(lcs_amd.LcsError):
         ctx.comb(np.zeros(10, np.int32), np.zeros(10, np.int32), lcs_amd.schedule(variant=7))
+
+
+@pytest.mark.parametrize("m,n,strips", [(0, 0, 0), (0, 5, 0), (7, 0, 0), (1, 1, 0), (65, 129, 0), (7330, 9944, 0),
+                                        (3000, 40000, 3), (3000, 40000, 80)])
+def test_permutation_built_on_the_gpu(ctx, m, n, strips):
+    """lcs_comb_permutation: the comb's strands turned into the permutation
+    on the device must equal PermutationMatrix::FromStrands of the strands
+    (the host routine, itself pinned to the oracle)."""
+    rng = np.random.default_rng(m * 31 + n + strips)
+    a = rng.integers(0, 4, m).astype(np.int32)
+    b = rng.integers(0, 4, n).astype(np.int32)
+    sched = lcs_amd.schedule(tile_m=4, strips=strips) if strips else None
+    h, v, llcs = ctx.comb(a, b, sched)
+    r2c, c2r, llcs2 = ctx.comb_permutation(a, b, sched)
+    r2c_o, c2r_o = lcs_amd.from_strands(h, v)
+    assert np.array_equal(r2c, r2c_o) and np.array_equal(c2r, c2r_o) and llcs2 == llcs
+    if m + n:
+        assert np.array_equal(np.sort(r2c), np.arange(m + n))
+
+
+def test_permutation_built_on_the_gpu_thin_pair_hash(ctx):
+    a = lcs_amd.gen_zipf(5, 1 << 12, 26, 1.1)
+    b = lcs_amd.gen_planted(6, a, 1 << 18, 26, 1.1, 16, 0.05)
+    r2c, c2r, llcs = ctx.comb_permutation(a, b)
+    ho, vo = oracle.comb(a, b, order="mt", threads=8)
+    assert lcs_amd.perm_hash(c2r) == oracle.perm_hash(oracle.from_strands(ho, vo)[1])
+    assert llcs == oracle.llcs_from_strands(vo, len(a))
