@@ -208,7 +208,7 @@ class Context:
                                         ctypes.byref(sched) if sched is not None else None,
                                         _p(r2c), _p(c2r), ctypes.byref(llcs))
         _check("lcs_comb_permutation", rc)
-        return r2c[:size].copy(), c2r[:size].copy(), llcs.value
+        return r2c[:size], c2r[:size], llcs.value
 
     def comb_fixup(self, a_prepared, b, h, v, i0, isize, j0, jsize):
         ap, b = _i32(a_prepared), _i32(b)
