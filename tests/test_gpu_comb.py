@@ -150,6 +150,19 @@ def test_thin_pair_eight_row_strips(ctx):
     assert np.array_equal(h, h1) and np.array_equal(v, v1) and llcs == llcs1
 
 
+def test_thin_pair_at_the_largest_strip_height(ctx):
+    # m = 32768, the largest height that is cut into strips: the merges run
+    # on the global-memory steady-ant core; one row more and the pair is
+    # combed as one strip
+    a = lcs_amd.gen_zipf(13, 1 << 15, 4, 1.1)
+    b = lcs_amd.gen_planted(14, a, 1 << 18, 4, 1.1, 4, 0.05)
+    h, v, llcs = ctx.comb(a, b)
+    h1, v1, llcs1 = ctx.comb(a, b, lcs_amd.schedule(strips=1))
+    assert np.array_equal(h, h1) and np.array_equal(v, v1) and llcs == llcs1
+    with pytest.raises(lcs_amd.LcsError):
+        ctx.comb(np.append(a, 1).astype(np.int32), b, lcs_amd.schedule(strips=2))
+
+
 LDS = 2  # LCS_VARIANT_CHAIN_LDS
 
 
