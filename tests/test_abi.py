@@ -79,3 +79,15 @@ def test_context_creation_fails_cleanly_without_gpu():
     with pytest.raises(lcs_amd.LcsError) as e:
         lcs_amd.Context(0)
     assert e.value.rc == -5
+
+
+def test_host_entry_points_reject_bad_arguments_before_touching_a_device():
+    # a null context or negative lengths are refused with LCS_EINVAL (-22)
+    # before any HIP call, so this runs without a GPU
+    lib = lcs_amd.lib()
+    a = np.zeros(4, np.int32)
+    out = np.zeros(8, np.int32)
+    p = a.ctypes.data_as(lcs_amd._i32p)
+    q = out.ctypes.data_as(lcs_amd._i32p)
+    assert lib.lcs_comb(None, p, 4, p, 4, None, q, q, None) == -22
+    assert lib.lcs_comb_permutation(None, p, 4, p, 4, None, q, q, None) == -22
