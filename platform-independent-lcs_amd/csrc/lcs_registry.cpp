@@ -215,6 +215,10 @@ LcsAlgorithmRegistry::LcsAlgorithmRegistry()
 {
     reg2["hip_auto"] = Lcs_Semi_Hip;
     schedules["hip_auto"] = lcs_schedule{0, 0, 0, 0};
+    // "tiled": the name the reference's own run scripts give its tiled comb
+    // (tests/test_prepare.ipynb), so those scripts run with device_type=gpu
+    reg2["tiled"] = Lcs_Semi_Hip;
+    schedules["tiled"] = lcs_schedule{0, 0, 0, 0};
     reg["hip_auto_fn"] = Lcs_Semi_Hip_Function;
     reg2["hip_llcs"] = Lcs_Llcs_Hip;
     score_only["hip_llcs"] = true;

@@ -27,13 +27,14 @@ def test_cli_script_matches_golden_hashes(tmp_path):
         "# a comment line\n"
         "algorithm=hip_tiled_4_1_2 device_type=gpu size_a=9771 size_b=11302 seed_a=888 seed_b=999\n"
         "algorithm=no_such_algorithm device_type=gpu size_a=10 size_b=10\n"
-        "algorithm=hip_tiled_16_1_1 device_type=gpu size_a=9771 size_b=11302 seed_a=888 seed_b=999\n")
+        "algorithm=hip_tiled_16_1_1 device_type=gpu size_a=9771 size_b=11302 seed_a=888 seed_b=999\n"
+        "algorithm=tiled device_type=gpu size_a=7330 size_b=9944 seed_a=888 seed_b=999\n")
     out = tmp_path / "out.csv"
     r = subprocess.run([CLI, str(script), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2, r.stderr          # one unknown algorithm
     assert "no_such_algorithm" in r.stderr
     rows = list(csv.DictReader(io.StringIO(out.read_text())))
-    assert [x["algorithm"] for x in rows] == ["hip_auto", "hip_auto", "hip_tiled_4_1_2", "hip_tiled_16_1_1"]
+    assert [x["algorithm"] for x in rows] == ["hip_auto", "hip_auto", "hip_tiled_4_1_2", "hip_tiled_16_1_1", "tiled"]
     g = golden()
     for row in rows:
         key = (int(row["size_a"]), int(row["size_b"]), 888, 999)
