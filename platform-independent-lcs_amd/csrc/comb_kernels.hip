@@ -16,8 +16,8 @@
 // b symbol move one lane down per step by DPP wave_shr:1; lane 0 receives the
 // band's inputs from two "conveyor" registers shifted by wave_shl:1, and the v
 // strand leaving lane 63 is caught by a third conveyor.  Nothing touches LDS.
-// Every 16 steps the wave publishes the 16 newest bottom-edge strands of its
-// band to a ring in global memory as 64-bit (tag, strand) words stored with
+// Every C steps (a chunk: 32 in the register chain, 16 in the LDS groups)
+// the wave publishes the C newest bottom-edge strands of its band to a ring in global memory as 64-bit (tag, strand) words stored with
 // device-scope relaxed atomics; the band below polls for words carrying its
 // tag, so no fence, flag or cache flush is needed.  Bands are handed out by an
 // atomic ticket in launch order, so a wave only ever waits for a band that a
@@ -675,15 +675,22 @@ __global__ void comb_identity_kernel(int32_t *h_out, int32_t m, int32_t *v_out, 
 
 using namespace lcsk;
 
-template <int TM, int C>
+// Steps per chunk.  The register chain takes 32: against 16, half the
+// per-chunk publish, prefetch and tag-check work outweighs the longer
+// band-to-band lag (m=n=2^20 at TM=16: 143.9 against 145.6 ms; 8 and 64
+// steps are slower).  The LDS groups keep 16.
+constexpr int kChainChunk = 32;
+constexpr int kGroupChunk = 16;
+
+template <int TM>
 static int launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_threads, hipStream_t stream)
 {
     if (args.group_w == 1) {
-        comb_chain_kernel<TM, C><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+        comb_chain_kernel<TM, kChainChunk><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
         return 0;
     }
     if (args.group_w == 4 && block_threads == 4 * kWave) {
-        comb_group_kernel<TM, C, 4><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
+        comb_group_kernel<TM, kGroupChunk, 4><<<dim3(grid_blocks), dim3(block_threads), 0, stream>>>(args);
         return 0;
     }
     return -22;
@@ -700,12 +707,12 @@ int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs
     if (args.ntasks > 0) {
         int rc;
         switch (tile_m) {
-        case 1: rc = launch_chain_tm<1, 16>(args, grid_blocks, block_threads, stream); break;
-        case 2: rc = launch_chain_tm<2, 16>(args, grid_blocks, block_threads, stream); break;
-        case 4: rc = launch_chain_tm<4, 16>(args, grid_blocks, block_threads, stream); break;
-        case 8: rc = launch_chain_tm<8, 16>(args, grid_blocks, block_threads, stream); break;
-        case 16: rc = launch_chain_tm<16, 16>(args, grid_blocks, block_threads, stream); break;
-        case 32: rc = launch_chain_tm<32, 16>(args, grid_blocks, block_threads, stream); break;
+        case 1: rc = launch_chain_tm<1>(args, grid_blocks, block_threads, stream); break;
+        case 2: rc = launch_chain_tm<2>(args, grid_blocks, block_threads, stream); break;
+        case 4: rc = launch_chain_tm<4>(args, grid_blocks, block_threads, stream); break;
+        case 8: rc = launch_chain_tm<8>(args, grid_blocks, block_threads, stream); break;
+        case 16: rc = launch_chain_tm<16>(args, grid_blocks, block_threads, stream); break;
+        case 32: rc = launch_chain_tm<32>(args, grid_blocks, block_threads, stream); break;
         default: rc = -22;
         }
         if (rc) return rc;

@@ -33,14 +33,16 @@ def test_edge_shapes(ctx, m, n):
 
 
 def test_interior_chunk_bounds(ctx):
-    """Widths around the band loop's phase boundaries: a chunk is 16 steps,
-    the interior (unmasked) chunks start at step 80 and stop two chunks
-    before n, and run two per iteration, so every n mod 32 and the widths
-    with zero, one and two interior chunks are covered, across several
-    bands."""
+    """Widths around the band loop's phase boundaries: a chunk of the
+    register chain is 32 steps, the interior (unmasked) chunks start at step
+    96 and stop two chunks before n, and run two per iteration, so every
+    n mod 64 and the widths with zero, one, two and three interior chunks
+    are covered, across several bands.  (The LDS groups' 16-step chunks are
+    covered by the group tests below.)"""
     rng = np.random.default_rng(404)
-    for n in list(range(76, 84)) + [95, 96, 97, 111, 112, 113, 127, 128, 129, 143, 144, 145, 159, 160, 161,
-                                    175, 176, 177, 191, 192, 193, 207, 208, 209, 1000, 1001, 1023, 1024, 1025]:
+    for n in list(range(76, 100)) + [111, 112, 113, 127, 128, 129, 143, 144, 145, 159, 160, 161, 175, 176, 177,
+                                     191, 192, 193, 207, 208, 209, 223, 224, 225, 255, 256, 257, 287, 288, 289,
+                                     319, 320, 321] + list(range(960, 1026)):
         a = rng.integers(0, 3, 2500).astype(np.int32)
         b = rng.integers(0, 3, n).astype(np.int32)
         for tm in (4, 16):
