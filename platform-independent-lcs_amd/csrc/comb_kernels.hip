@@ -16,8 +16,8 @@
 // b symbol move one lane down per step by DPP wave_shr:1; lane 0 receives the
 // band's inputs from two "conveyor" registers shifted by wave_shl:1, and the v
 // strand leaving lane 63 is caught by a third conveyor.  Nothing touches LDS.
-// Every C steps (a chunk: 32 in the register chain, 16 in the LDS groups)
-// the wave publishes the C newest bottom-edge strands of its band to a ring in global memory as 64-bit (tag, strand) words stored with
+// Every C = 32 steps (a chunk) the wave publishes the C newest bottom-edge
+// strands of its band to a ring in global memory as 64-bit (tag, strand) words stored with
 // device-scope relaxed atomics; the band below polls for words carrying its
 // tag, so no fence, flag or cache flush is needed.  Bands are handed out by an
 // atomic ticket in launch order, so a wave only ever waits for a band that a
@@ -675,12 +675,12 @@ __global__ void comb_identity_kernel(int32_t *h_out, int32_t m, int32_t *v_out, 
 
 using namespace lcsk;
 
-// Steps per chunk.  The register chain takes 32: against 16, half the
-// per-chunk publish, prefetch and tag-check work outweighs the longer
-// band-to-band lag (m=n=2^20 at TM=16: 143.9 against 145.6 ms; 8 and 64
-// steps are slower).  The LDS groups keep 16.
+// Steps per chunk.  32 for both forms: against 16, half the per-chunk
+// publish, prefetch and tag-check work outweighs the longer band-to-band lag
+// (m=n=2^20 at TM=16: the register chain 143.9 against 145.6 ms, the LDS
+// groups 154.7 against 166.0 ms; 8 and 64 steps are slower in both).
 constexpr int kChainChunk = 32;
-constexpr int kGroupChunk = 16;
+constexpr int kGroupChunk = 32;
 
 template <int TM>
 static int launch_chain_tm(const ChainArgs &args, int grid_blocks, int block_threads, hipStream_t stream)
