@@ -613,8 +613,8 @@ struct InvertBatch {
 __global__ void invert_kernel(InvertBatch B)
 {
     const auto &S = B.s[blockIdx.y];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < S.n && (uint32_t)S.r2c[i] < (uint32_t)S.n) S.c2r[S.r2c[i]] = i;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // no wrap near 2^31
+    if (i < S.n && (uint32_t)S.r2c[i] < (uint32_t)S.n) S.c2r[S.r2c[i]] = (int32_t)i;
 }
 
 struct AssembleBatch {
@@ -626,7 +626,7 @@ struct AssembleBatch {
 __global__ void assemble_kernel(AssembleBatch B)
 {
     const AssembleArgs &A = B.a[blockIdx.y];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // no wrap near 2^31
     const int k = A.k, ps = A.p_size, qs = A.q_size;
     // the range tests only matter after an aborted comb fed undefined input
     if (i < k && (uint32_t)A.core[i] < (uint32_t)k) {

@@ -599,15 +599,16 @@ __global__ __launch_bounds__(kWave * W) void comb_group_kernel(ChainArgs A)
 __global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, int nctrl)
 {
     const PairDesc P = pairs[blockIdx.y];
-    const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride) {
+    // 64-bit indices: a grid-stride step past a size near 2^31 must not wrap
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride) {
         const uint32_t v0 = P.v_in ? (uint32_t)P.v_in[c] : (uint32_t)(P.m + c);
         P.E[c] = (unsigned long long)v0;                       // tag 0
         P.E[(size_t)P.n + c] = 0xffffffff00000000ull;          // no band has this tag
     }
     if (P.nbands == 0)
-        for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P.m; r += stride)
-            P.h_out[r] = P.h_in ? P.h_in[r] : r;
+        for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < P.m; r += stride)
+            P.h_out[r] = P.h_in ? P.h_in[r] : (int32_t)r;
     if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nctrl) ctrl[threadIdx.x] = 0u;
 }
 
@@ -616,8 +617,9 @@ __global__ void comb_extract_kernel(const PairDesc *pairs)
 {
     const PairDesc P = pairs[blockIdx.y];
     const unsigned long long *slot = P.E + (size_t)(P.nbands & 1) * (size_t)P.n;
-    const int stride = gridDim.x * blockDim.x;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride) P.v_out[c] = (int32_t)(uint32_t)slot[c];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.n; c += stride)
+        P.v_out[c] = (int32_t)(uint32_t)slot[c];
 }
 
 // row_to_col of each pair's permutation (blockIdx.y = pair): the strand that
@@ -627,13 +629,13 @@ __global__ void from_strands_kernel(const PairDesc *pairs, int32_t *perm, const 
 {
     const PairDesc P = pairs[blockIdx.y];
     int32_t *r2c = perm + perm_off[blockIdx.y];
-    const int stride = gridDim.x * blockDim.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
     const uint32_t size = (uint32_t)(P.m + P.n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.m + P.n; i += stride) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)size; i += stride) {
         // ids are always < m+n after a completed comb; the test keeps an
         // aborted one (whose strands are undefined) from writing out of range
         const uint32_t id = (uint32_t)(i < P.m ? P.h_out[i] : P.v_out[i - P.m]);
-        if (id < size) r2c[id] = i < P.m ? P.n + i : i - P.m;
+        if (id < size) r2c[id] = (int32_t)(i < P.m ? P.n + i : i - P.m);
     }
 }
 
@@ -641,34 +643,34 @@ __global__ void from_strands_kernel(const PairDesc *pairs, int32_t *perm, const 
 // src/permutation_matrix.hpp:88-115): h[l] -> n+l, v[j] -> j.
 __global__ void r2c_from_strands_kernel(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t size = (uint32_t)(m + n);
-    if (i >= m + n) return;
+    if (i >= (long long)size) return;
     // ids are always < m+n after a completed comb; the test keeps an aborted
     // one (whose strands are undefined) from writing out of range
     const uint32_t id = (uint32_t)(i < m ? h[i] : v[i - m]);
-    if (id < size) r2c[id] = i < m ? n + i : i - m;
+    if (id < size) r2c[id] = (int32_t)(i < m ? n + i : i - m);
 }
 
 __global__ void invert_perm_kernel(const int32_t *r2c, int32_t size, int32_t *c2r)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < size && (uint32_t)r2c[i] < (uint32_t)size) c2r[r2c[i]] = i;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < size && (uint32_t)r2c[i] < (uint32_t)size) c2r[r2c[i]] = (int32_t)i;
 }
 
 __global__ void strands_from_c2r_kernel(const int32_t *c2r, int32_t m, int32_t n, int32_t *h_out, int32_t *v_out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v_out[i] = c2r[i];
-    else if (i < n + m) h_out[i - n] = c2r[i];
+    else if (i < (long long)n + m) h_out[i - n] = c2r[i];
 }
 
 // Identity strands for empty grids (m == 0 or n == 0).
 __global__ void comb_identity_kernel(int32_t *h_out, int32_t m, int32_t *v_out, int32_t n)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) h_out[i] = i;
-    if (i < n) v_out[i] = m + i;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) h_out[i] = (int32_t)i;
+    if (i < n) v_out[i] = (int32_t)(m + i);
 }
 
 }  // namespace lcsk
