@@ -85,7 +85,9 @@ int lcs_debug_band_trace(lcs_ctx *ctx, int32_t max_bands, uint64_t *host_out);
  * (src/lcs_tiled.hpp:95-137) together with its residual fixup
  * (src/lcs_grid.hpp:451-493): host pointers, synchronous.  h_out has m
  * entries, v_out n entries; llcs_out (nullable) receives #{ j : v_out[j] < m }.
- * sched may be NULL. */
+ * sched may be NULL.  Sizes: m + n <= 2^31 - 1 and n <= 2^31 - 4097 (the
+ * widest b one band chain combs), else LCS_ERANGE; the same limits hold for
+ * every comb entry point below. */
 int lcs_comb(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_t n,
              const lcs_schedule *sched, int32_t *h_out, int32_t *v_out, int32_t *llcs_out);
 
@@ -196,7 +198,9 @@ int lcs_braid_multiply(lcs_ctx *ctx, const int32_t *p_row_to_col, int32_t p_size
  * alphabet of at most 200 symbols.  Symbols of a and b must lie in
  * [0, sigma), 1 <= sigma <= 200; a symbol outside gives LCS_ERANGE (host
  * entry points) or a length of -1 (device entry point).  Equal to
- * #{v_out[j] < m} of lcs_comb on the same pair.
+ * #{v_out[j] < m} of lcs_comb on the same pair.  The string combed as
+ * columns (the longer one, as a rule) may hold at most 2^31 - 4097
+ * symbols, else LCS_ERANGE.
  *   lcs_llcs: host pointers, synchronous.
  *   lcs_llcs_device: device pointers, asynchronous on the context's stream;
  *     *d_llcs is a device int32.

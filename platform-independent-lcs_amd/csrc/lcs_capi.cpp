@@ -115,6 +115,8 @@ int group_width(int variant)
     }
 }
 
+constexpr int32_t kMaxChainN = 0x7fffffff - 4096;   // widest b a band chain combs
+
 int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
                   DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w)
 {
@@ -125,6 +127,8 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     int32_t max_bands = 0, max_n = 0;
     unsigned long long *E = ring;
     for (auto &P : pairs) {
+        // the band loop counts steps up to n + 64 + 2 chunks in int
+        if (P.n > kMaxChainN) return LCS_ERANGE;
         P.tile_m = tm;
         P.nbands = bands_of(P.m, P.n, tm);
         if (ring) {  // else every pair already carries its own ring
@@ -297,6 +301,7 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
         }
         return lcsk_launch_identity(d_h, m, d_v, n, stream);
     }
+    if (n > kMaxChainN) return LCS_ERANGE;   // before any buffer is reserved
     const int strips_req = sched ? sched->strips : 0;
     int tm = (sched && sched->tile_m) ? sched->tile_m : 0;
     if (tm && !valid_tile_m(tm)) return LCS_EINVAL;

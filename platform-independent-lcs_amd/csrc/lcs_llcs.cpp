@@ -55,6 +55,8 @@ int enqueue_llcs(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma
         D.cols = swap ? P.a : P.b;
         D.nrows = swap ? P.n : P.m;
         D.ncols = swap ? P.m : P.n;
+        // the band loop counts columns up to ncols + 3 chunks in int
+        if (D.ncols > 0x7fffffff - 4096) return LCS_ERANGE;
         D.nbands = lcsl::llcs_nbands(D.nrows);
         D.nchunks = lcsl::llcs_nchunks(D.ncols);
         D.out = P.out;
