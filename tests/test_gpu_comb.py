@@ -37,8 +37,8 @@ def test_interior_chunk_bounds(ctx):
     register chain is 32 steps, the interior (unmasked) chunks start at step
     96 and stop two chunks before n, and run two per iteration, so every
     n mod 64 and the widths with zero, one, two and three interior chunks
-    are covered, across several bands.  (The LDS groups' 16-step chunks are
-    covered by the group tests below.)"""
+    are covered, across several bands, for the register chain and for the
+    LDS band groups (same chunk length, their own loop)."""
     rng = np.random.default_rng(404)
     for n in list(range(76, 100)) + [111, 112, 113, 127, 128, 129, 143, 144, 145, 159, 160, 161, 175, 176, 177,
                                      191, 192, 193, 207, 208, 209, 223, 224, 225, 255, 256, 257, 287, 288, 289,
@@ -47,6 +47,7 @@ def test_interior_chunk_bounds(ctx):
         b = rng.integers(0, 3, n).astype(np.int32)
         for tm in (4, 16):
             _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=4, variant=LDS))
 
 
 @pytest.mark.parametrize("tm", [1, 2, 4, 8, 16, 32])
