@@ -37,6 +37,8 @@ def _load():
     lib.orc_comb_antidiag.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p]
     lib.orc_comb_rows.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p]
     lib.orc_comb_mt.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p, i32]
+    lib.orc_comb_simd.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p, i32, i32]
+    lib.orc_comb_simd.restype = i32
     lib.orc_fixup.argtypes = [_i32p, _i32p, _i32p, _i32p, i32, i32, i32, i32]
     lib.orc_from_strands.argtypes = [_i32p, i32, _i32p, i32, _i32p, _i32p]
     lib.orc_hash.argtypes = [_i32p, i32]
@@ -74,8 +76,20 @@ def _arr(x) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
 
 
-def comb(a, b, order: str = "rows", threads: int = 0):
-    """Strands (h_out[m], v_out[n]) of the seaweed comb of a against b."""
+ISA_AUTO, ISA_AVX2, ISA_AVX512, ISA_SCALAR = 0, 1, 2, 3
+
+
+def simd_isa_available(isa: int) -> bool:
+    """Whether this host can run orc_comb_simd with `isa`."""
+    h, v = np.empty(1, np.int32), np.empty(1, np.int32)
+    a = np.zeros(1, np.int32)
+    return lib().orc_comb_simd(_c(a), 1, _c(a), 1, _c(h), _c(v), 1, isa) >= 0
+
+
+def comb(a, b, order: str = "rows", threads: int = 0, isa: int = ISA_AUTO):
+    """Strands (h_out[m], v_out[n]) of the seaweed comb of a against b.
+    order: "rows" (sequential), "antidiag" (the reference's order), "mt"
+    (scalar block wavefront) or "simd" (vectorised stripes, `isa`)."""
     a, b = _arr(a), _arr(b)
     m, n = a.size, b.size
     h = np.empty(max(m, 1), np.int32)
@@ -84,6 +98,10 @@ def comb(a, b, order: str = "rows", threads: int = 0):
         lib().orc_comb_antidiag(_c(a), m, _c(b), n, _c(h), _c(v))
     elif order == "mt":
         lib().orc_comb_mt(_c(a), m, _c(b), n, _c(h), _c(v), threads or os.cpu_count() or 1)
+    elif order == "simd":
+        used = lib().orc_comb_simd(_c(a), m, _c(b), n, _c(h), _c(v), threads or os.cpu_count() or 1, isa)
+        if used < 0:
+            raise RuntimeError(f"orc_comb_simd: this host cannot run isa {isa}")
     else:
         lib().orc_comb_rows(_c(a), m, _c(b), n, _c(h), _c(v))
     return h[:m].copy(), v[:n].copy()

@@ -15,6 +15,10 @@
  *         no-match grids: identity permutation when nothing matches);
  *       - bijection on [0, m+n); a == b gives LCS = m;
  *       - the three orders agree with each other (order independence).
+ *   orc_comb_simd  - equals orc_comb_rows bit for bit at every stripe edge
+ *                    (8/16 lanes x 8 rows), column-block edge (2048), empty
+ *                    grids, negative and 32-bit symbols, all-match/no-match,
+ *                    for each instruction set the host has and 1..8 threads.
  *   orc_fixup      - combing the four quadrants in dependency order through
  *                    the fixup equals the whole comb.
  *   orc_from_strands / orc_hash
@@ -37,8 +41,10 @@
  *                    strings included); dominance counts equal numpy's count
  *                    of the definition; out-of-range queries give -1.
  */
+#define _GNU_SOURCE
 #include "lcs_oracle.h"
 
+#include <sched.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -147,6 +153,232 @@ void orc_comb_mt(const int32_t *a, int32_t m, const int32_t *b, int32_t n,
         }
     }
     free(ar);
+}
+
+/* ------------------------------------------------------ vectorised comb */
+/*
+ * orc_comb_simd: the reference's tiled multithreaded comb on CPU SIMD lanes
+ * (Lcs_Semi_Tiled_MT, lcs_tiled.hpp:95-137; update_stripe,
+ * lcs_common.hpp:131-173, whose sub-group lanes each own TILE_M rows).
+ *
+ * Grid orientation: row r holds a[r] (the reference's reversed row m-1-r),
+ * v strands flow from row r-1 into row r, row r starts with strand m-1-r and
+ * column c with m+c.  A stripe is R = W*TM consecutive rows; lane l keeps
+ * rows l*TM .. l*TM+TM-1 of it (symbols and h strands) in registers.  At
+ * step t lane l combs column t-l, top row first; the v strand then moves one
+ * lane down (lane shift), lane 0 takes the stripe's input v of column t and
+ * lane W-1 hands the stripe's output for column t-W+1 on.  v lives in a
+ * column-reversed array (vrev[P + n-1-c]), b likewise, so lane l's symbol at
+ * step t is one contiguous load.  Rows past m never match and hold strand 0,
+ * the smallest id, so every v strand passes them unchanged.
+ *
+ * Stripes are dealt round-robin to the threads; a thread combs its stripe in
+ * column blocks of CB, each after the stripe above has finished that block
+ * (a per-stripe progress counter, release/acquire).  Ramp steps at a block's
+ * two ends mask the lanes whose column lies outside the block.
+ */
+#include <immintrin.h>
+#include <stdatomic.h>
+
+#define SIMD_TM 8
+#define SIMD_PAD 64
+#define SIMD_CB 2048
+
+typedef struct {
+    const int32_t *at;      /* [stripe][k][lane] symbols                    */
+    uint32_t *ht;           /* [stripe][k][lane] h strands                  */
+    const int32_t *brev;    /* brev[SIMD_PAD + n-1-c] = b[c]                */
+    uint32_t *vrev;         /* vrev[SIMD_PAD + n-1-c] = v of column c       */
+    int32_t m, n, nst, W;
+} simd_grid;
+
+/* active lanes of a step: j0 <= t - l < j1 */
+static inline uint32_t ramp_mask(int64_t t, int64_t j0, int64_t j1, int W)
+{
+    int64_t lo = t - j1 + 1, hi = t - j0;
+    if (lo < 0) lo = 0;
+    if (hi > W - 1) hi = W - 1;
+    if (hi < lo) return 0;
+    return (uint32_t)((((uint64_t)1 << (hi + 1)) - 1) & ~(((uint64_t)1 << lo) - 1));
+}
+
+__attribute__((target("avx512f"))) static void
+stripe_block_avx512(const simd_grid *G, int32_t I, int32_t j0, int32_t j1)
+{
+    enum { W = 16, TM = SIMD_TM };
+    const int32_t n = G->n;
+    const int32_t *at = G->at + (size_t)I * TM * W;
+    uint32_t *ht = G->ht + (size_t)I * TM * W;
+    __m512i A[TM], H[TM];
+    __mmask16 valid[TM];
+    const int rowmask = (int64_t)(I + 1) * W * TM > G->m;
+#pragma GCC unroll 16
+    for (int k = 0; k < TM; ++k) {
+        A[k] = _mm512_loadu_si512((const void *)(at + k * W));
+        H[k] = _mm512_loadu_si512((const void *)(ht + k * W));
+        uint32_t vm = 0;
+        for (int l = 0; l < W; ++l) vm |= (uint32_t)((int64_t)I * W * TM + l * TM + k < G->m) << l;
+        valid[k] = (__mmask16)vm;
+    }
+    const __m512i ones = _mm512_set1_epi32(-1);
+    __m512i V = _mm512_setzero_si512();
+    const int32_t *bp = G->brev + SIMD_PAD + n - 1;
+    uint32_t *vp = G->vrev + SIMD_PAD + n - 1;
+    const int64_t t_end = (int64_t)j1 + W - 1;
+    for (int64_t t = j0; t < t_end; ++t) {
+        const uint32_t vin = t < j1 ? vp[-t] : 0u;
+        V = _mm512_alignr_epi32(V, _mm512_set1_epi32((int)vin), W - 1);
+        const __m512i B = _mm512_loadu_si512((const void *)(bp - t));
+        const int full = t >= (int64_t)j0 + W - 1 && t < j1;
+        if (full && !rowmask) {
+#pragma GCC unroll 16
+            for (int k = 0; k < TM; ++k) {
+                const __mmask16 mt = _mm512_cmpeq_epi32_mask(A[k], B);
+                const __m512i hn = _mm512_min_epu32(_mm512_mask_mov_epi32(H[k], mt, ones), V);
+                V = _mm512_ternarylogic_epi32(H[k], V, hn, 0x96);
+                H[k] = hn;
+            }
+        } else {
+            const __mmask16 act = (__mmask16)ramp_mask(t, j0, j1, W);
+#pragma GCC unroll 16
+            for (int k = 0; k < TM; ++k) {
+                const __mmask16 mt = _mm512_cmpeq_epi32_mask(A[k], B) & valid[k];
+                const __m512i hn = _mm512_min_epu32(_mm512_mask_mov_epi32(H[k], mt, ones), V);
+                V = _mm512_ternarylogic_epi32(H[k], V, hn, 0x96);
+                H[k] = _mm512_mask_mov_epi32(H[k], act, hn);
+            }
+        }
+        /* lane W-1 leaves the stripe with column t-W+1 */
+        if (t - (W - 1) >= j0 && t - (W - 1) < j1) _mm512_mask_storeu_epi32((void *)(vp - t), (__mmask16)0x8000, V);
+    }
+#pragma GCC unroll 16
+    for (int k = 0; k < TM; ++k) _mm512_storeu_si512((void *)(ht + k * W), H[k]);
+}
+
+__attribute__((target("avx2"))) static void
+stripe_block_avx2(const simd_grid *G, int32_t I, int32_t j0, int32_t j1)
+{
+    enum { W = 8, TM = SIMD_TM };
+    const int32_t n = G->n;
+    const int32_t *at = G->at + (size_t)I * TM * W;
+    uint32_t *ht = G->ht + (size_t)I * TM * W;
+    __m256i A[TM], H[TM], valid[TM];
+#pragma GCC unroll 16
+    for (int k = 0; k < TM; ++k) {
+        A[k] = _mm256_loadu_si256((const __m256i *)(at + k * W));
+        H[k] = _mm256_loadu_si256((const __m256i *)(ht + k * W));
+        int32_t vm[8];
+        for (int l = 0; l < W; ++l) vm[l] = (int64_t)I * W * TM + l * TM + k < G->m ? -1 : 0;
+        valid[k] = _mm256_loadu_si256((const __m256i *)vm);
+    }
+    const int rowmask = (int64_t)(I + 1) * W * TM > G->m;
+    const __m256i rot = _mm256_setr_epi32(7, 0, 1, 2, 3, 4, 5, 6);
+    __m256i V = _mm256_setzero_si256();
+    const int32_t *bp = G->brev + SIMD_PAD + n - 1;
+    uint32_t *vp = G->vrev + SIMD_PAD + n - 1;
+    const int64_t t_end = (int64_t)j1 + W - 1;
+    for (int64_t t = j0; t < t_end; ++t) {
+        const uint32_t vin = t < j1 ? vp[-t] : 0u;
+        V = _mm256_blend_epi32(_mm256_permutevar8x32_epi32(V, rot), _mm256_set1_epi32((int)vin), 0x01);
+        const __m256i B = _mm256_loadu_si256((const __m256i *)(bp - t));
+        const int full = t >= (int64_t)j0 + W - 1 && t < j1;
+        if (full && !rowmask) {
+#pragma GCC unroll 16
+            for (int k = 0; k < TM; ++k) {
+                const __m256i hx = _mm256_or_si256(H[k], _mm256_cmpeq_epi32(A[k], B));
+                const __m256i hn = _mm256_min_epu32(hx, V);
+                V = _mm256_xor_si256(_mm256_xor_si256(H[k], V), hn);
+                H[k] = hn;
+            }
+        } else {
+            const uint32_t am = ramp_mask(t, j0, j1, W);
+            int32_t act_l[8];
+            for (int l = 0; l < W; ++l) act_l[l] = (am >> l) & 1 ? -1 : 0;
+            const __m256i act = _mm256_loadu_si256((const __m256i *)act_l);
+#pragma GCC unroll 16
+            for (int k = 0; k < TM; ++k) {
+                const __m256i mt = _mm256_and_si256(_mm256_cmpeq_epi32(A[k], B), valid[k]);
+                const __m256i hn = _mm256_min_epu32(_mm256_or_si256(H[k], mt), V);
+                V = _mm256_xor_si256(_mm256_xor_si256(H[k], V), hn);
+                H[k] = _mm256_blendv_epi8(H[k], hn, act);
+            }
+        }
+        if (t - (W - 1) >= j0 && t - (W - 1) < j1) vp[-(t - (W - 1))] = (uint32_t)_mm256_extract_epi32(V, 7);
+    }
+#pragma GCC unroll 16
+    for (int k = 0; k < TM; ++k) _mm256_storeu_si256((__m256i *)(ht + k * W), H[k]);
+}
+
+int32_t orc_comb_simd(const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+                      int32_t *h_out, int32_t *v_out, int32_t threads, int32_t isa)
+{
+    __builtin_cpu_init();
+    const int has512 = __builtin_cpu_supports("avx512f"), has2 = __builtin_cpu_supports("avx2");
+    if (isa == 0) isa = has512 ? 2 : has2 ? 1 : 3;
+    if ((isa == 2 && !has512) || (isa == 1 && !has2) || isa < 1 || isa > 3) return -1;
+    if (threads < 1) threads = 1;
+    if (isa == 3 || m == 0 || n == 0) {
+        orc_comb_mt(a, m, b, n, h_out, v_out, threads);
+        return isa;
+    }
+    const int W = isa == 2 ? 16 : 8, TM = SIMD_TM, R = W * TM;
+    const int32_t nst = (int32_t)(((int64_t)m + R - 1) / R);
+    const size_t cells = (size_t)nst * R;
+    int32_t *at = (int32_t *)malloc(sizeof(int32_t) * cells);
+    uint32_t *ht = (uint32_t *)malloc(sizeof(uint32_t) * cells);
+    int32_t *brev = (int32_t *)calloc((size_t)n + 2 * SIMD_PAD, sizeof(int32_t));
+    uint32_t *vrev = (uint32_t *)calloc((size_t)n + 2 * SIMD_PAD, sizeof(uint32_t));
+    const int32_t ncb = (int32_t)(((int64_t)n + SIMD_CB - 1) / SIMD_CB);
+    atomic_int *progress = (atomic_int *)malloc(sizeof(atomic_int) * (size_t)nst);
+    for (int32_t I = 0; I < nst; ++I)
+        for (int k = 0; k < TM; ++k)
+            for (int l = 0; l < W; ++l) {
+                const int64_t r = (int64_t)I * R + (int64_t)l * TM + k;
+                const size_t x = ((size_t)I * TM + k) * W + l;
+                at[x] = r < m ? a[r] : 0;
+                ht[x] = r < m ? (uint32_t)(m - 1 - r) : 0u;
+            }
+    for (int32_t c = 0; c < n; ++c) {
+        brev[SIMD_PAD + n - 1 - c] = b[c];
+        vrev[SIMD_PAD + n - 1 - c] = (uint32_t)(m + c);
+    }
+    for (int32_t I = 0; I < nst; ++I) atomic_init(&progress[I], 0);
+    const simd_grid G = {at, ht, brev, vrev, m, n, nst, W};
+#pragma omp parallel num_threads(threads)
+    {
+        int tid = 0, nth = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        for (int32_t I = tid; I < nst; I += nth)
+            for (int32_t J = 0; J < ncb; ++J) {
+                if (I > 0) {
+                    long spins = 0;
+                    while (atomic_load_explicit(&progress[I - 1], memory_order_acquire) <= J) {
+                        _mm_pause();
+                        if (++spins > 4096) { sched_yield(); spins = 0; }
+                    }
+                }
+                const int32_t j0 = J * SIMD_CB, j1 = j0 + SIMD_CB < n ? j0 + SIMD_CB : n;
+                if (isa == 2) stripe_block_avx512(&G, I, j0, j1);
+                else stripe_block_avx2(&G, I, j0, j1);
+                atomic_store_explicit(&progress[I], J + 1, memory_order_release);
+            }
+    }
+    for (int32_t I = 0; I < nst; ++I)
+        for (int k = 0; k < TM; ++k)
+            for (int l = 0; l < W; ++l) {
+                const int64_t r = (int64_t)I * R + (int64_t)l * TM + k;
+                if (r < m) h_out[m - 1 - r] = (int32_t)ht[((size_t)I * TM + k) * W + l];
+            }
+    for (int32_t c = 0; c < n; ++c) v_out[c] = (int32_t)vrev[SIMD_PAD + n - 1 - c];
+    free(at);
+    free(ht);
+    free(brev);
+    free(vrev);
+    free(progress);
+    return isa;
 }
 
 /* lcs_residual_fixup.hpp:74-122 (Lcs_Semi_Fixup). */

@@ -42,6 +42,17 @@ void orc_comb_rows(const int32_t *a, int32_t m, const int32_t *b, int32_t n,
 void orc_comb_mt(const int32_t *a, int32_t m, const int32_t *b, int32_t n,
                  int32_t *h_out, int32_t *v_out, int32_t threads);
 
+/* Vectorised multithreaded comb, the CPU shape of the reference's tiled
+ * comb (Lcs_Semi_Tiled_MT, lcs_tiled.hpp:95-137, with update_stripe,
+ * lcs_common.hpp:131-173): the W lanes of a SIMD register own W*TM rows of a
+ * stripe and sweep its columns along skewed anti-diagonals; stripes and
+ * column blocks run as a pipeline over `threads` threads.  isa: 0 = best the
+ * host offers, 1 = AVX2 (W = 8), 2 = AVX-512 (W = 16), 3 = scalar
+ * (orc_comb_mt).  Returns the isa used, or -1 if the host lacks the one
+ * asked for.  Same result as orc_comb_rows. */
+int32_t orc_comb_simd(const int32_t *a, int32_t m, const int32_t *b, int32_t n,
+                      int32_t *h_out, int32_t *v_out, int32_t threads, int32_t isa);
+
 /* Comb a sub-rectangle given prepared (reversed) a, b and current strands,
  * the reference's residual fixup (lcs_residual_fixup.hpp:74-122). */
 void orc_fixup(const int32_t *a_prepared, const int32_t *b_prepared,

@@ -62,6 +62,38 @@ def test_equal_strings_and_orders_agree():
         assert np.array_equal(r[0], o[0]) and np.array_equal(r[1], o[1])
 
 
+SIMD_ISAS = [i for i in (oracle.ISA_AVX2, oracle.ISA_AVX512) if oracle.simd_isa_available(i)]
+
+
+@pytest.mark.parametrize("isa", SIMD_ISAS)
+def test_simd_comb_equals_rows(isa):
+    """The vectorised stripe comb (the CPU baseline and the checker of the
+    full-size pairs) against the sequential comb: stripe edges (8 or 16 lanes
+    of 8 rows), column-block edges (2048), empty grids, negative and 32-bit
+    symbols, all-match and no-match, 1..8 threads (the pipeline over threads)."""
+    rng = np.random.default_rng(11)
+    W = 16 if isa == oracle.ISA_AVX512 else 8
+    edges = [0, 1, 7, 8, 9, W - 1, W, W + 1, 8 * W - 1, 8 * W, 8 * W + 1, 2047, 2048, 2049]
+    cases = [(m, n) for m in edges for n in (0, 1, 5, W + 1, 2049)]
+    cases += [(int(rng.integers(0, 3000)), int(rng.integers(0, 6000))) for _ in range(40)]
+    for k, (m, n) in enumerate(cases):
+        sigma = int(rng.integers(1, 6))
+        a, b = rnd(1000 + k, m, n, sigma)
+        if k % 7 == 3:   # negative and 32-bit symbols
+            a = (a.astype(np.int64) * 0x3fffffff - 0x40000000).astype(np.int32)
+            b = (b.astype(np.int64) * 0x3fffffff - 0x40000000).astype(np.int32)
+        ho, vo = oracle.comb(a, b)
+        for th in (1, 3, 8):
+            h, v = oracle.comb(a, b, order="simd", threads=th, isa=isa)
+            assert np.array_equal(h, ho) and np.array_equal(v, vo), (m, n, sigma, th)
+    # all-match and no-match grids
+    z, o = np.zeros(333, np.int32), np.ones(4100, np.int32)
+    for a, b in ((z, z[:100]), (o, o), (z, o)):
+        ho, vo = oracle.comb(a, b)
+        h, v = oracle.comb(a, b, order="simd", threads=4, isa=isa)
+        assert np.array_equal(h, ho) and np.array_equal(v, vo)
+
+
 def test_semilocal_windows_equal_dp():
     for seed in range(25):
         rng = np.random.default_rng(100 + seed)
