@@ -45,7 +45,7 @@ extern "C" {
 typedef struct lcs_ctx lcs_ctx;
 
 /* A combing schedule.  Any field set to 0 means "choose automatically"
- * (autotuner table, then the learned cost model, then the built-in default).
+ * (the built-in defaults).
  * Schedules only change speed: every schedule gives the same bits. */
 typedef struct lcs_schedule {
     int32_t tile_m;          /* rows per lane: 1, 2, 4, 8, 16 or 32          */
@@ -71,6 +71,24 @@ int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len);
 /* Number of compute units, and the kernel-measured time in milliseconds of
  * the last lcs_comb / lcs_comb_device call (HIP events around the kernels). */
 int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms);
+/* Waits for every kernel the context has enqueued (its own stream and the
+ * streams given to the asynchronous entry points) and reports whether any
+ * launch since the previous lcs_ctx_status gave up a device-side wait:
+ * LCS_ETIMEDOUT (the outputs of that launch are undefined), else LCS_OK.
+ * The asynchronous entry points (lcs_comb_device, lcs_llcs_device) return
+ * after enqueueing, so this is how their callers learn of an abort.  The
+ * report is cleared by the call.  No counterpart in the reference, whose
+ * calls are synchronous (src/algorithm_registry.cpp:43-75). */
+int lcs_ctx_status(lcs_ctx *ctx);
+/* Bound of every device-side wait, in ticks of the 100 MHz device clock
+ * (default 500000000 = 5 s).  A wait that exceeds it makes its launch give
+ * up: every wave exits and the call (or lcs_ctx_status) reports
+ * LCS_ETIMEDOUT.  0 makes every wait that finds its data not yet there give
+ * up at once (a way to exercise the error path). */
+int lcs_ctx_set_timeout(lcs_ctx *ctx, uint64_t ticks);
+/* Device bytes the context holds now (workspaces grow on demand and stay
+ * until lcs_ctx_destroy) and its budget. */
+int lcs_ctx_memory(lcs_ctx *ctx, size_t *held, size_t *budget);
 
 /* Diagnostics (no counterpart in the reference): record the start and end
  * time (100 MHz device clock) of each band of the following chain launches
@@ -103,7 +121,12 @@ int lcs_comb_permutation(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_
 
 /* Same computation on device pointers, enqueued on `stream` (a hipStream_t;
  * NULL = the context's stream).  Returns after enqueueing; the caller
- * synchronises the stream.  Workspace comes from the context. */
+ * synchronises the stream and calls lcs_ctx_status to learn whether the
+ * launch completed.  Workspace (ring, descriptors, control words) comes
+ * from the context and is shared by all its calls: a call on a stream other
+ * than the previous call's first waits, on the device, for the previous
+ * call's kernels, so calls through one context never overlap.  Concurrent
+ * combs take one context per stream, or lcs_batch_comb. */
 int lcs_comb_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
                     const lcs_schedule *sched, int32_t *d_h_out, int32_t *d_v_out, void *stream);
 

@@ -36,6 +36,7 @@ struct ChainArgs {
     unsigned long long *trace;   // nullable: [2t] start, [2t+1] end of task t (realtime ticks)
     int32_t trace_cap;           // tasks the trace buffer holds
     int32_t pad1;
+    unsigned *sticky;            // nullable: the prep kernel ORs the abort flag into it before clearing it
 };
 
 // Seeds the rings of pairs[0..npairs) and clears ticket/abort; then runs the

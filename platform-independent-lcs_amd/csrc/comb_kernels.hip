@@ -594,9 +594,11 @@ __global__ __launch_bounds__(kWave * W) void comb_group_kernel(ChainArgs A)
 // Seeds the rings before a chain launch (blockIdx.y = pair): slot 0 holds
 // the grid's top edge (v_in[c], or the ids m+c) under tag 0 for band 0,
 // slot 1 holds a tag no band waits for.  A pair with no band (n == 0) gets
-// its h_out here.  Block (0,0) clears the ticket and abort words.  A kernel,
-// not hipMemsetAsync, so the launch sequence can live inside a hipGraph.
-__global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, int nctrl)
+// its h_out here.  Block (0,0) clears the ticket and abort words, after
+// folding the previous launch's abort flag into `sticky` (the context's
+// status word, read and cleared by the host).  A kernel, not
+// hipMemsetAsync, so the launch sequence can live inside a hipGraph.
+__global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, unsigned *sticky)
 {
     const PairDesc P = pairs[blockIdx.y];
     // 64-bit indices: a grid-stride step past a size near 2^31 must not wrap
@@ -609,7 +611,11 @@ __global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, int nctr
     if (P.nbands == 0)
         for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < P.m; r += stride)
             P.h_out[r] = P.h_in ? P.h_in[r] : (int32_t)r;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nctrl) ctrl[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (sticky && ctrl[1]) *sticky = 1u;
+        ctrl[0] = 0u;
+        ctrl[1] = 0u;
+    }
 }
 
 // v_out = the strands the last band left in its slot (blockIdx.y = pair).
@@ -702,10 +708,9 @@ int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs
                       int grid_blocks, int block_threads, hipStream_t stream)
 {
     if (npairs <= 0) return 0;
-    int fill_blocks = (max_n + 255) / 256;
-    if (fill_blocks < 1) fill_blocks = 1;
-    if (fill_blocks > 1024) fill_blocks = 1024;
-    hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs, args.ticket, 2);
+    const int fill_blocks = (int)std::min<long long>(1024, std::max<long long>(1, ((long long)max_n + 255) / 256));
+    hipLaunchKernelGGL(comb_prep_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs, args.ticket,
+                       args.sticky);
     if (args.ntasks > 0) {
         int rc;
         switch (tile_m) {
@@ -727,9 +732,7 @@ int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, i
                              const int64_t *d_perm_off, hipStream_t stream)
 {
     if (npairs <= 0) return 0;
-    int blocks = (max_len + 255) / 256;
-    if (blocks < 1) blocks = 1;
-    if (blocks > 1024) blocks = 1024;
+    const int blocks = (int)std::min<long long>(1024, std::max<long long>(1, ((long long)max_len + 255) / 256));
     hipLaunchKernelGGL(from_strands_kernel, dim3(blocks, npairs), dim3(256), 0, stream, d_pairs, perm, d_perm_off);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
@@ -737,27 +740,29 @@ int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, i
 int lcsk_launch_r2c_from_strands(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c,
                                  hipStream_t stream)
 {
-    const int size = m + n;
+    const long long size = (long long)m + n;
     if (size == 0) return 0;
-    hipLaunchKernelGGL(r2c_from_strands_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, h, m, v, n, r2c);
+    hipLaunchKernelGGL(r2c_from_strands_kernel, dim3((unsigned)((size + 255) / 256)), dim3(256), 0, stream, h, m, v, n,
+                       r2c);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
 int lcsk_launch_strands_from_perm(const int32_t *r2c, int32_t m, int32_t n, int32_t *scratch, int32_t *h_out,
                                   int32_t *v_out, hipStream_t stream)
 {
-    const int size = m + n;
+    const long long size = (long long)m + n;
     if (size == 0) return 0;
-    hipLaunchKernelGGL(invert_perm_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, r2c, size, scratch);
-    hipLaunchKernelGGL(strands_from_c2r_kernel, dim3((size + 255) / 256), dim3(256), 0, stream, scratch, m, n, h_out,
-                       v_out);
+    const dim3 grid((unsigned)((size + 255) / 256));
+    hipLaunchKernelGGL(invert_perm_kernel, grid, dim3(256), 0, stream, r2c, (int32_t)size, scratch);
+    hipLaunchKernelGGL(strands_from_c2r_kernel, grid, dim3(256), 0, stream, scratch, m, n, h_out, v_out);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
 int lcsk_launch_identity(int32_t *h_out, int32_t m, int32_t *v_out, int32_t n, hipStream_t stream)
 {
-    const int mx = m > n ? m : n;
+    const long long mx = m > n ? m : n;
     if (mx == 0) return 0;
-    hipLaunchKernelGGL(comb_identity_kernel, dim3((mx + 255) / 256), dim3(256), 0, stream, h_out, m, v_out, n);
+    hipLaunchKernelGGL(comb_identity_kernel, dim3((unsigned)((mx + 255) / 256)), dim3(256), 0, stream, h_out, m, v_out,
+                       n);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }

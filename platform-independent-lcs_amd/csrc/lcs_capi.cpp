@@ -157,6 +157,7 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     A.ntasks = (int32_t)tasks.size();
     A.ticket = ctrl;
     A.abort_flag = ctrl + 1;
+    A.sticky = ctx->ctrl + kStickyWord;
     A.timeout_ticks = ctx->timeout_ticks;
     A.trace = ctx->band_trace;
     A.trace_cap = ctx->band_trace_cap;
@@ -288,10 +289,39 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
     return lcsk_launch_strands_from_perm(cur[0], m, n, c2r, d_h, d_v, stream);
 }
 
-// Enqueue one pair on `stream` with device pointers, following `sched`.
+int ws_acquire(lcs_ctx *ctx, hipStream_t s)
+{
+    if (ctx->ws_stream && ctx->ws_stream != s && hipStreamWaitEvent(s, ctx->ws_done, 0) != hipSuccess) return LCS_EIO;
+    return LCS_OK;
+}
+
+int ws_release(lcs_ctx *ctx, hipStream_t s)
+{
+    if (hipEventRecord(ctx->ws_done, s) != hipSuccess) return LCS_EIO;
+    ctx->ws_stream = s;
+    return LCS_OK;
+}
+
+static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
+                            const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
+                            const int32_t *d_h_in, const int32_t *d_v_in);
+
+// Enqueue one pair on `stream` with device pointers, following `sched`,
+// after the previous user of the context's workspace.
 int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
                   const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
                   const int32_t *d_h_in, const int32_t *d_v_in)
+{
+    int rc = ws_acquire(ctx, stream);
+    if (rc) return rc;
+    rc = enqueue_chain_ws(ctx, d_a, m, d_b, n, sched, d_h, d_v, stream, d_h_in, d_v_in);
+    const int rc2 = ws_release(ctx, stream);
+    return rc ? rc : rc2;
+}
+
+static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
+                            const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
+                            const int32_t *d_h_in, const int32_t *d_v_in)
 {
     if (m == 0 || n == 0) {
         if (d_h_in || d_v_in) {
@@ -339,11 +369,17 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
     return enqueue_pairs(ctx, pairs, tm, wps, stream, group_w);
 }
 
+// The abort flag of the last launch on the context's own ctrl words (the
+// caller has synchronised).  A raised flag is cleared once reported, so the
+// next launch's prep kernel does not fold it into the status word again.
 int check_abort(lcs_ctx *ctx)
 {
     unsigned flag = 0;
     if (hipMemcpy(&flag, ctx->ctrl + 1, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess) return LCS_EIO;
-    return flag ? LCS_ETIMEDOUT : LCS_OK;
+    if (!flag) return LCS_OK;
+    const unsigned zero = 0;
+    if (hipMemcpy(ctx->ctrl + 1, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) return LCS_EIO;
+    return LCS_ETIMEDOUT;
 }
 
 }  // namespace lcs
@@ -376,7 +412,8 @@ int lcs_ctx_create(int device, size_t hbm_budget, lcs_ctx **out)
     ctx->budget = hbm_budget ? hbm_budget : free_b / 4 * 3;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return LCS_EIO; }
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ws_done, hipEventDisableTiming) != hipSuccess) {
         lcs_ctx_destroy(ctx);
         return LCS_EIO;
     }
@@ -399,6 +436,7 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     ctx->strip_ws.release(ctx);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
+    if (ctx->ws_done) (void)hipEventDestroy(ctx->ws_done);
     ctx->in_a.release(ctx);
     ctx->in_b.release(ctx);
     ctx->out_h.release(ctx);
@@ -417,6 +455,39 @@ int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len)
 {
     if (!ctx || !buf || len == 0) return LCS_EINVAL;
     std::snprintf(buf, len, "%s (%s)", ctx->name.c_str(), ctx->arch.c_str());
+    return LCS_OK;
+}
+
+int lcs_ctx_status(lcs_ctx *ctx)
+{
+    if (!ctx) return LCS_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return LCS_EIO;
+    if (ctx->ws_stream && hipEventSynchronize(ctx->ws_done) != hipSuccess) return LCS_EIO;
+    unsigned words[64];
+    if (hipMemcpy(words, ctx->ctrl, sizeof(words), hipMemcpyDeviceToHost) != hipSuccess) return LCS_EIO;
+    const bool aborted = words[1] || words[kStickyWord];
+    if (aborted) {
+        const unsigned zero = 0;
+        if (hipMemcpy(ctx->ctrl + 1, &zero, 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ctx->ctrl + kStickyWord, &zero, 4, hipMemcpyHostToDevice) != hipSuccess)
+            return LCS_EIO;
+    }
+    return aborted ? LCS_ETIMEDOUT : LCS_OK;
+}
+
+int lcs_ctx_memory(lcs_ctx *ctx, size_t *held, size_t *budget)
+{
+    if (!ctx) return LCS_EINVAL;
+    if (held) *held = ctx->held;
+    if (budget) *budget = ctx->budget;
+    return LCS_OK;
+}
+
+int lcs_ctx_set_timeout(lcs_ctx *ctx, uint64_t ticks)
+{
+    if (!ctx) return LCS_EINVAL;
+    ctx->timeout_ticks = ticks;
     return LCS_OK;
 }
 

@@ -28,6 +28,12 @@ struct PreparedLaunch {
 
 }  // namespace lcs
 
+namespace lcs {
+// ctrl word that collects the abort flags of finished launches (folded in by
+// each launch's prep kernel); lcs_ctx_status reads and clears it
+constexpr int kStickyWord = 63;
+}  // namespace lcs
+
 struct lcs_ctx {
     int device = 0;
     int num_cus = 0;
@@ -42,6 +48,11 @@ struct lcs_ctx {
     bool pending_timing = false;
     lcs::DevBuf ring, in_a, in_b, out_h, out_v, descs, strip_ws, perm;
     hipEvent_t upload_done = nullptr;    // last copy out of `pinned`
+    // The shared workspace (ring, descriptors, ctrl words) is used by one
+    // launch at a time: a call on another stream first waits for ws_done,
+    // recorded after the last call's kernels on ws_stream.
+    hipEvent_t ws_done = nullptr;
+    hipStream_t ws_stream = nullptr;
     void *pinned = nullptr;              // pinned staging for descriptor uploads
     size_t pinned_bytes = 0;
     lcs::Extras *extras = nullptr;
@@ -56,6 +67,10 @@ int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_
                   const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
                   const int32_t *d_h_in = nullptr, const int32_t *d_v_in = nullptr);
 int check_abort(lcs_ctx *ctx);
+// order a call on `s` after the previous user of the shared workspace, and
+// mark `s` as its latest user once the call's kernels are enqueued
+int ws_acquire(lcs_ctx *ctx, hipStream_t s);
+int ws_release(lcs_ctx *ctx, hipStream_t s);
 int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream);
 size_t pairs_ring_words(const std::vector<PairDesc> &pairs);
 // group_w: 1 = register chain (one wave per band), 4 = LDS groups of 4 bands

@@ -38,7 +38,18 @@ int resident_bands(const lcs_ctx *ctx) { return 4 * ctx->num_cus; }
 // Enqueues one launch over `pairs` on `s`: the longer-pipelined orientation
 // is chosen per pair (LCS is symmetric).  Rings and descriptors live in the
 // context's buffers, error words after the descriptors.
+static int enqueue_llcs_ws(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma, hipStream_t s);
+
 int enqueue_llcs(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma, hipStream_t s)
+{
+    int rc = ws_acquire(ctx, s);
+    if (rc) return rc;
+    rc = enqueue_llcs_ws(ctx, pairs, sigma, s);
+    const int rc2 = ws_release(ctx, s);
+    return rc ? rc : rc2;
+}
+
+static int enqueue_llcs_ws(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma, hipStream_t s)
 {
     const int np = (int)pairs.size();
     if (np == 0) return LCS_OK;
@@ -89,6 +100,7 @@ int enqueue_llcs(lcs_ctx *ctx, const std::vector<LlcsPair> &pairs, int32_t sigma
     A.sigma = sigma;
     A.ticket = ctx->ctrl;
     A.abort_flag = ctx->ctrl + 1;
+    A.sticky = ctx->ctrl + kStickyWord;
     A.timeout_ticks = ctx->timeout_ticks;
     A.trace = ctx->band_trace;
     A.trace_cap = ctx->band_trace_cap;

@@ -40,6 +40,7 @@ struct LlcsArgs {
     unsigned long long *trace;   // nullable: [2t] start, [2t+1] end of task t (realtime ticks)
     int32_t trace_cap;
     int32_t pad;
+    unsigned *sticky;            // nullable: the prep kernel ORs the abort flag into it before clearing it
 };
 
 constexpr int kLdsSymRing = 2048, kLdsInRing = 128;

@@ -456,8 +456,9 @@ __global__ __launch_bounds__(2 * kWave) void llcs_chain_kernel(LlcsArgs A)
     }
 }
 
-// grid (x: ring words, y: pair); block (0,0) clears the ticket and abort words
-__global__ void llcs_prep_kernel(const LlcsDesc *descs, unsigned *ticket, unsigned *abort_flag)
+// grid (x: ring words, y: pair); block (0,0) clears the ticket and abort
+// words, after folding the previous launch's abort flag into `sticky`
+__global__ void llcs_prep_kernel(const LlcsDesc *descs, unsigned *ticket, unsigned *abort_flag, unsigned *sticky)
 {
     const LlcsDesc D = descs[blockIdx.y];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,6 +468,7 @@ __global__ void llcs_prep_kernel(const LlcsDesc *descs, unsigned *ticket, unsign
         *D.err = 0;
     }
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (sticky && *abort_flag) *sticky = 1u;
         *ticket = 0u;
         *abort_flag = 0u;
     }
@@ -488,9 +490,9 @@ int lcsl_launch(const lcsl::LlcsArgs &args, const lcsl::LlcsDesc *d_descs, int n
     using namespace lcsl;
     if (npairs <= 0) return 0;
     if (args.sigma < 1 || args.sigma > kMaxSigma) return -22;
-    const unsigned px = (unsigned)((max_ring_words + 255) / 256);
+    const unsigned px = (unsigned)(((long long)max_ring_words + 255) / 256);
     hipLaunchKernelGGL(llcs_prep_kernel, dim3(px > 0 ? px : 1, npairs), dim3(256), 0, stream, d_descs, args.ticket,
-                       args.abort_flag);
+                       args.abort_flag, args.sticky);
     if (args.ntasks > 0 && grid_blocks > 0)
         hipLaunchKernelGGL(llcs_chain_kernel, dim3(grid_blocks), dim3(2 * kWave), llcs_lds_bytes(args.sigma), stream,
                            args);

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "liblcs_hip.so")
 LIB_PATH = os.environ.get("LCS_HIP_LIB", LIB_PATH)
 
 LCS_OK = 0
+LCS_ETIMEDOUT = -110
+DEFAULT_TIMEOUT_TICKS = 500_000_000   # 5 s of the 100 MHz device clock
 ERRORS = {
     -2: "ENOENT", -5: "EIO", -11: "EAGAIN", -12: "ENOMEM", -16: "EBUSY", -22: "EINVAL",
     -34: "ERANGE", -110: "ETIMEDOUT", -125: "ECANCELED",
@@ -68,6 +70,9 @@ def lib() -> ctypes.CDLL:
     L.lcs_ctx_destroy.argtypes = [vp]
     L.lcs_ctx_device_name.argtypes = [vp, ctypes.c_char_p, sz]
     L.lcs_ctx_info.argtypes = [vp, _i32p, ctypes.POINTER(ctypes.c_double)]
+    L.lcs_ctx_status.argtypes = [vp]
+    L.lcs_ctx_set_timeout.argtypes = [vp, ctypes.c_uint64]
+    L.lcs_ctx_memory.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     L.lcs_comb.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
     L.lcs_comb_permutation.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
     L.lcs_comb_device.argtypes = [vp, vp, i32, vp, i32, sched_p, vp, vp, vp]
@@ -183,6 +188,19 @@ class Context:
         ms = ctypes.c_double()
         _check("lcs_ctx_info", lib().lcs_ctx_info(self._h, ctypes.byref(cus), ctypes.byref(ms)))
         return cus.value, ms.value
+
+    def status(self) -> int:
+        """lcs_ctx_status: waits for the context's kernels; LCS_OK or LCS_ETIMEDOUT."""
+        return lib().lcs_ctx_status(self._h)
+
+    def set_timeout(self, ticks: int) -> None:
+        _check("lcs_ctx_set_timeout", lib().lcs_ctx_set_timeout(self._h, int(ticks)))
+
+    def memory(self):
+        """(held, budget) device bytes."""
+        held, budget = ctypes.c_size_t(), ctypes.c_size_t()
+        _check("lcs_ctx_memory", lib().lcs_ctx_memory(self._h, ctypes.byref(held), ctypes.byref(budget)))
+        return held.value, budget.value
 
     def comb(self, a, b, sched: Schedule | None = None):
         """(h_out, v_out, llcs) of the seaweed comb of a against b."""

@@ -25,6 +25,9 @@ def hip() -> ctypes.CDLL:
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipSetDevice.argtypes = [ctypes.c_int]
+        _hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
     return _hip
 
 
@@ -69,3 +72,19 @@ class DeviceArray:
 
 def synchronize() -> None:
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A HIP stream (default priority)."""
+
+    def __init__(self):
+        self.ptr = ctypes.c_void_p()
+        _ok(hip().hipStreamCreate(ctypes.byref(self.ptr)), "hipStreamCreate")
+
+    def synchronize(self) -> None:
+        _ok(hip().hipStreamSynchronize(self.ptr), "hipStreamSynchronize")
+
+    def close(self) -> None:
+        if self.ptr:
+            hip().hipStreamDestroy(self.ptr)
+            self.ptr = ctypes.c_void_p()

@@ -14,6 +14,12 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
 
 
+def cpu_threads() -> int:
+    """Host threads the CPU checker uses: this process's CPU share, at most
+    16 (the host share of one GPU on the test machines)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 @pytest.fixture(scope="session")
 def ctx():
     import lcs_amd

@@ -49,7 +49,5 @@ def test_budget_admission_and_errors(ctx):
     outs = [srv.wait(i, timeout_ms=60000) for i in ids]
     h1, v1, _ = ctx.comb(a, a[::-1].copy())
     assert all(np.array_equal(h, h1) and np.array_equal(v, v1) for h, v in outs)
-    with pytest.raises(lcs_amd.LcsError) as e:
-        srv.poll(10**9) if False else lcs_amd._check("x", lcs_amd.lib().lcs_server_cancel(srv._h, 10**9))
-    assert e.value.rc == -2
+    assert lcs_amd.lib().lcs_server_cancel(srv._h, 10**9) == -2   # unknown request id: LCS_ENOENT
     srv.close()
