@@ -54,29 +54,60 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(a, b, target_s: float):
-    """The oracle's multithreaded comb (a port of the reference's block
-    wavefront) on a full-height strip of the same pair, scaled to GCUPS."""
-    import oracle  # test infrastructure: the CPU baseline only
-    threads = min(16, len(os.sched_getaffinity(0)))
-    m = len(a)
-    # calibrate on a small strip, then size the sample for ~target_s seconds
-    cols = 256
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, cut to the cgroup's
+    CPU quota when one is set (cpu.max), so no thread waits for a share."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    return (min(n, quota) if quota else n), n, quota
+
+
+def _strip_rate(fn, m: int, b, target_s: float, min_cols: int):
+    """Times fn(b[:cols]) on a full-height strip sized for ~target_s."""
+    cols = min(len(b), min_cols)
     t0 = time.perf_counter()
-    oracle.comb(a, b[:cols], order="mt", threads=threads)
+    fn(b[:cols])
     rate = m * cols / max(time.perf_counter() - t0, 1e-6)
-    cols = int(min(len(b), max(512, rate * target_s / m)))
+    cols = int(min(len(b), max(min_cols, rate * target_s / m)))
     t0 = time.perf_counter()
-    oracle.comb(a, b[:cols], order="mt", threads=threads)
-    dt = time.perf_counter() - t0
-    # one core: the sequential restatement on a narrower full-height strip
-    cols1 = max(64, cols // (4 * threads))
-    t1 = time.perf_counter()
-    oracle.comb(a, b[:cols1], order="rows")
-    dt1 = time.perf_counter() - t1
+    fn(b[:cols])
+    return cols, time.perf_counter() - t0
+
+
+def cpu_baseline(a, b, target_s: float):
+    """The reference's CPU path restated in the oracle, timed on a
+    full-height strip of the same pair and scaled to GCUPS:
+      - value: the vectorised tiled comb (orc_comb_simd: the reference's
+        Lcs_Semi_Tiled_MT shape, lcs_tiled.hpp:95-137, with the lanes of a
+        sub-group owning rows, lcs_common.hpp:131-173, on AVX-512/AVX2 lanes)
+        on every CPU this process may use;
+      - scalar_block_wavefront: the scalar block wavefront (orc_comb_mt) on
+        the same threads;
+      - single_core: the sequential restatement of lcs_reference.hpp:5-69."""
+    import oracle  # test infrastructure: the CPU baseline only
+    threads, affinity, quota = host_cpus()
+    m = len(a)
+    cols, dt = _strip_rate(lambda bb: oracle.comb(a, bb, order="simd", threads=threads), m, b, target_s, 1024)
+    cols_mt, dt_mt = _strip_rate(lambda bb: oracle.comb(a, bb, order="mt", threads=threads), m, b,
+                                 target_s / 3, 256)
+    cols1, dt1 = _strip_rate(lambda bb: oracle.comb(a, bb, order="rows"), m, b, target_s / 6, 16)
+    isa = {1: "AVX2", 2: "AVX-512"}.get(2 if oracle.simd_isa_available(oracle.ISA_AVX512) else
+                                        1 if oracle.simd_isa_available(oracle.ISA_AVX2) else 3, "scalar")
     return {"value": round(m * cols / dt / 1e9, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+            "host_cpus": {"affinity": affinity, "cgroup_quota": quota},
             "sample": f"m={m} x first {cols} columns of the same pair ({m * cols:.3g} cells, {dt:.1f} s), "
-                      f"oracle orc_comb_mt block wavefront, {threads} threads",
+                      f"oracle orc_comb_simd ({isa} stripes, lanes own rows), {threads} threads",
+            "scalar_block_wavefront": {
+                "value": round(m * cols_mt / dt_mt / 1e9, 4), "unit": "GCUPS", "cores": threads,
+                "sample": f"m={m} x first {cols_mt} columns ({m * cols_mt:.3g} cells, {dt_mt:.1f} s), "
+                          "oracle orc_comb_mt (scalar block wavefront)"},
             "single_core": {"value": round(m * cols1 / dt1 / 1e9, 4), "unit": "GCUPS", "cores": 1,
                             "sample": f"m={m} x first {cols1} columns ({m * cols1:.3g} cells, {dt1:.1f} s), "
                                       "oracle orc_comb_rows (sequential restatement of lcs_reference.hpp:5-69)"}}
@@ -176,6 +207,9 @@ def bench_batch(ctx, npairs: int):
             t1 = time.perf_counter()
             lcs_amd.batch_comb_device(ctx, lay, da.ptr, db.ptr, dh1.ptr, dv1.ptr, scheds)
             dt = time.perf_counter() - t1
+            st = ctx.status()
+            if st:
+                raise lcs_amd.LcsError("lcs_ctx_status", st)
             if dt < best:
                 best, kms = dt, ctx.info()[1]
         exact = bool(np.array_equal(dh1.numpy(), ref_h) and np.array_equal(dv1.numpy(), ref_v))
@@ -224,6 +258,9 @@ def bench_zipf(ctx, steps: int = 3):
         if rc:
             raise lcs_amd.LcsError("lcs_comb_device", rc)
         H.synchronize()
+        st = ctx.status()
+        if st:
+            raise lcs_amd.LcsError("lcs_ctx_status", st)
 
     run(None)
     t0 = time.perf_counter()
@@ -377,6 +414,11 @@ def main():
         if rc:
             raise lcs_amd.LcsError("lcs_comb_device", rc)
         H.synchronize()
+        # every timed step: a device-side wait that gave up would end the
+        # launch early and be timed as a fast step
+        st = ctx.status()
+        if st:
+            raise lcs_amd.LcsError("lcs_ctx_status", st)
         return ctx.info()[1]
 
     for _ in range(args.warmup):

@@ -44,9 +44,13 @@ extern "C" {
 
 typedef struct lcs_ctx lcs_ctx;
 
-/* A combing schedule.  Any field set to 0 means "choose automatically"
- * (the built-in defaults).
- * Schedules only change speed: every schedule gives the same bits. */
+/* A combing schedule.  Schedules only change speed: every schedule gives
+ * the same bits.  A pair whose schedule is NULL or has tile_m == 0 gets one
+ * chosen by the context (lcs_ctx_choose_schedule, in this order): the
+ * context's autotune table, then its learned cost model, then the
+ * exhaustive autotune of the shape when enabled, then the built-in
+ * defaults; the caller's other non-zero fields are kept.  A field still 0
+ * after that takes the built-in default. */
 typedef struct lcs_schedule {
     int32_t tile_m;          /* rows per lane: 1, 2, 4, 8, 16 or 32          */
     int32_t waves_per_simd;  /* resident waves per SIMD the launch aims at   */
@@ -269,6 +273,51 @@ int lcs_batch_comb_device(lcs_ctx *ctx, int32_t npairs,
  * cost_ms has ncand entries. */
 int lcs_autotune_measure(lcs_ctx *ctx, int32_t m, int32_t n, int32_t alphabet, float mutation_rate, uint64_t seed,
                          const lcs_schedule *candidates, int32_t ncand, int32_t mode, int32_t reps, float *cost_ms);
+
+/* ---- schedule selection ---------------------------------------------------
+ * No counterpart in the reference.  Two modes, each with its own table,
+ * model and on-miss autotune: LCS_TUNE_SINGLE prices one pair alone
+ * (lcs_comb, lcs_comb_permutation, lcs_comb_device), LCS_TUNE_BATCH a pair
+ * inside a batch (lcs_batch_comb*).  For a pair (m, n) left to the library:
+ *   1. table: the autotuned shape nearest to (m, n) within 1/8 octave in
+ *      both lengths gives its measured best schedule;
+ *   2. model: the attached cost model (lcs_mlp, created on this context)
+ *      predicts a cost per candidate from (log2 m, log2 n, log2 alphabet,
+ *      mutation rate) and the cheapest candidate is taken.  The alphabet
+ *      is 1 + the largest symbol for host strings, 4 for device strings;
+ *      the mutation rate is taken as 0.1 (the comb's cost does not depend
+ *      on the symbols).  Decisions are memoised per (m, n, alphabet);
+ *   3. autotune on a miss (when enabled): lcs_autotune_measure times the
+ *      on-miss candidates on a synthetic pair of the shape (seconds) and
+ *      the best joins the table;
+ *   4. the built-in defaults.
+ * Table rows usually come from the autotuner's recorded tables (the
+ * Python package loads them); the model is trained on the same timings. */
+#define LCS_TUNE_SINGLE 0
+#define LCS_TUNE_BATCH 1
+#define LCS_SCHED_FROM_DEFAULT 0
+#define LCS_SCHED_FROM_TABLE 1
+#define LCS_SCHED_FROM_MODEL 2
+#define LCS_SCHED_FROM_AUTOTUNE 3
+#define LCS_SCHED_FROM_CALLER 4
+/* Adds (or replaces) the table row of shape (m, n); best->tile_m != 0. */
+int lcs_ctx_add_tuned(lcs_ctx *ctx, int32_t mode, int32_t m, int32_t n, const lcs_schedule *best);
+int lcs_ctx_clear_tuned(lcs_ctx *ctx, int32_t mode);
+/* Attaches a cost model whose n_out outputs price candidates[0..ncand)
+ * (n_out == ncand, every tile_m != 0); NULL detaches.  The model must have
+ * been created on this context and stays owned by the caller;
+ * lcs_mlp_destroy detaches it. */
+struct lcs_mlp;
+int lcs_ctx_set_cost_model(lcs_ctx *ctx, int32_t mode, struct lcs_mlp *model, const lcs_schedule *candidates,
+                           int32_t ncand);
+/* Enables the exhaustive autotune of shapes that the table and the model
+ * leave open (ncand = 0 disables it); reps timed runs per candidate. */
+int lcs_ctx_set_autotune_on_miss(lcs_ctx *ctx, int32_t mode, const lcs_schedule *candidates, int32_t ncand,
+                                 int32_t reps);
+/* The schedule the context would use for a pair of this shape, and which
+ * step chose it (LCS_SCHED_FROM_*). */
+int lcs_ctx_choose_schedule(lcs_ctx *ctx, int32_t mode, int32_t m, int32_t n, int32_t alphabet, float rate,
+                            lcs_schedule *out, int32_t *source);
 
 /* ---- request server ------------------------------------------------------
  * No counterpart in the reference.  Continuous batching of pair comparisons:

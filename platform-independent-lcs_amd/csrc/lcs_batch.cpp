@@ -187,6 +187,24 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
     return LCS_OK;
 }
 
+// Schedules of a batch's pairs: those left to the library come from the
+// context's batch-mode selection (table, model, autotune on a miss,
+// default).  Host strings give the model each pair's alphabet.
+int batch_schedules(lcs_ctx *ctx, int32_t npairs, const int32_t *m, const int32_t *n, const int32_t *a_host,
+                    const int64_t *a_off, const lcs_schedule *scheds, std::vector<lcs_schedule> &out)
+{
+    const Selector &S = ctx->select[LCS_TUNE_BATCH];
+    if (S.table.empty() && !S.model && S.miss_cands.empty()) {   // nothing to choose with: keep the caller's
+        out.assign((size_t)npairs, lcs_schedule{});
+        if (scheds) std::copy(scheds, scheds + npairs, out.begin());
+        return LCS_OK;
+    }
+    std::vector<int32_t> alphabet((size_t)npairs, 4);
+    if (a_host && (S.model || !S.miss_cands.empty()))
+        for (int32_t p = 0; p < npairs; ++p) alphabet[(size_t)p] = alphabet_of(a_host + a_off[p], m[p]);
+    return choose_schedules(ctx, LCS_TUNE_BATCH, npairs, m, n, alphabet.data(), scheds, out);
+}
+
 int check_batch_args(int32_t npairs, const int64_t *a_off, const int32_t *m, const int64_t *b_off,
                      const int32_t *n, int64_t &total_a, int64_t &total_b)
 {
@@ -214,6 +232,9 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     if (rc) return rc;
     if ((total_a && (!a_all || !h_out)) || (total_b && (!b_all || !v_out))) return LCS_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    // before any batch buffer is touched: an autotune on a miss runs batches itself
+    std::vector<lcs_schedule> resolved;
+    if ((rc = batch_schedules(ctx, npairs, m, n, a_all, a_off, scheds, resolved))) return rc;
     Extras *X = get_extras(ctx);
     if (!X) return LCS_ENOMEM;
     if ((rc = X->batch_a.reserve(ctx, (size_t)std::max<int64_t>(total_a, 1) * 4))) return rc;
@@ -225,7 +246,7 @@ extern "C" int lcs_batch_comb(lcs_ctx *ctx, int32_t npairs, const int32_t *a_all
     hipStream_t s = ctx->stream;
     if (total_a && hipMemcpyAsync(da, a_all, (size_t)total_a * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
     if (total_b && hipMemcpyAsync(db, b_all, (size_t)total_b * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LCS_EIO;
-    if ((rc = batch_core(ctx, X, npairs, da, a_off, m, db, b_off, n, scheds, use_graph, dh, dv))) return rc;
+    if ((rc = batch_core(ctx, X, npairs, da, a_off, m, db, b_off, n, resolved.data(), use_graph, dh, dv))) return rc;
     if (total_a && hipMemcpyAsync(h_out, dh, (size_t)total_a * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
     if (total_b && hipMemcpyAsync(v_out, dv, (size_t)total_b * 4, hipMemcpyDeviceToHost, s) != hipSuccess) return LCS_EIO;
     if (hipStreamSynchronize(s) != hipSuccess) return LCS_EIO;
@@ -244,7 +265,10 @@ extern "C" int lcs_batch_comb_device(lcs_ctx *ctx, int32_t npairs, const int32_t
     if (rc) return rc;
     if ((total_a && (!d_a_all || !d_h_out)) || (total_b && (!d_b_all || !d_v_out))) return LCS_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    std::vector<lcs_schedule> resolved;
+    if ((rc = batch_schedules(ctx, npairs, m, n, nullptr, a_off, scheds, resolved))) return rc;
     Extras *X = get_extras(ctx);
     if (!X) return LCS_ENOMEM;
-    return batch_core(ctx, X, npairs, d_a_all, a_off, m, d_b_all, b_off, n, scheds, use_graph, d_h_out, d_v_out);
+    return batch_core(ctx, X, npairs, d_a_all, a_off, m, d_b_all, b_off, n, resolved.data(), use_graph, d_h_out,
+                      d_v_out);
 }

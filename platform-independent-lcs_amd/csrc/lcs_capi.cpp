@@ -71,33 +71,49 @@ int32_t bands_of(int32_t m, int32_t n, int tm)
     return (int32_t)(((long long)m + 64LL * tm - 1) / (64LL * tm));
 }
 
-// Stages host bytes into the context's pinned upload buffer and copies them
-// to `dst` on `stream`.  The previous upload is waited for first, so the
-// pinned buffer is never rewritten while a copy from it is pending.
-int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream)
+int Staging::init()
+{
+    return hipEventCreateWithFlags(&done, hipEventDisableTiming) == hipSuccess ? LCS_OK : LCS_EIO;
+}
+
+void Staging::destroy()
+{
+    if (done) (void)hipEventSynchronize(done);
+    if (pinned) (void)hipHostFree(pinned);
+    if (done) (void)hipEventDestroy(done);
+    pinned = nullptr;
+    bytes = 0;
+    done = nullptr;
+}
+
+// Stages host bytes into a pinned buffer and copies them to `dst` on
+// `stream`.  The previous upload through the same staging is waited for
+// first, so the buffer is never rewritten while a copy from it is pending.
+int upload_via(Staging &st, void *dst, const void *src, size_t bytes, hipStream_t stream)
 {
     if (bytes == 0) return LCS_OK;
-    if (hipEventSynchronize(ctx->upload_done) != hipSuccess) return LCS_EIO;
-    if (bytes > ctx->pinned_bytes) {
-        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-        ctx->pinned = nullptr;
-        ctx->pinned_bytes = 0;
+    if (hipEventSynchronize(st.done) != hipSuccess) return LCS_EIO;
+    if (bytes > st.bytes) {
+        if (st.pinned) (void)hipHostFree(st.pinned);
+        st.pinned = nullptr;
+        st.bytes = 0;
         void *p = nullptr;
         if (hipHostMalloc(&p, bytes * 2, hipHostMallocDefault) != hipSuccess) return LCS_ENOMEM;
-        ctx->pinned = p;
-        ctx->pinned_bytes = bytes * 2;
+        st.pinned = p;
+        st.bytes = bytes * 2;
     }
-    std::memcpy(ctx->pinned, src, bytes);
-    if (hipMemcpyAsync(dst, ctx->pinned, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) return LCS_EIO;
-    if (hipEventRecord(ctx->upload_done, stream) != hipSuccess) return LCS_EIO;
+    std::memcpy(st.pinned, src, bytes);
+    if (hipMemcpyAsync(dst, st.pinned, bytes, hipMemcpyHostToDevice, stream) != hipSuccess) return LCS_EIO;
+    if (hipEventRecord(st.done, stream) != hipSuccess) return LCS_EIO;
     return LCS_OK;
 }
 
-// Prepares one chain launch over `pairs` (one tile height): assigns each
-// pair its ring inside `ring` (which must hold pairs_ring_words(pairs)
-// words), lists the tasks band-major (all pairs start at once, and every
-// band comes after the band above it) and uploads descriptors and tasks to
-// `descs`.  Fills `L` for launch_prepared().
+int upload(lcs_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t stream)
+{
+    return upload_via(ctx->staging, dst, src, bytes, stream);
+}
+
+// Ring words of a launch over `pairs` (one ring of lcsk_ring_words(n) each).
 size_t pairs_ring_words(const std::vector<PairDesc> &pairs)
 {
     size_t w = 0;
@@ -117,8 +133,12 @@ int group_width(int variant)
 
 constexpr int32_t kMaxChainN = 0x7fffffff - 4096;   // widest b a band chain combs
 
-int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
-                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w)
+// One chain launch over `pairs` (one tile height): assigns each pair its
+// ring inside `ring` (which must hold pairs_ring_words(pairs) words; NULL
+// when every pair carries its own) and lists the tasks band-major (all pairs
+// start at once, and every band comes after the band above it).
+int build_launch(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
+                 unsigned *ctrl, int group_w, std::vector<unsigned char> &blob, PreparedLaunch &L)
 {
     if (!valid_tile_m(tm) || wps < 1 || wps > 8 || (group_w != 1 && group_w != 4)) return LCS_EINVAL;
     const int np = (int)pairs.size();
@@ -145,15 +165,13 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
         for (int p = 0; p < np; ++p)
             if (g < (pairs[(size_t)p].nbands + group_w - 1) / group_w) tasks.push_back(make_int2(p, g));
     const size_t pair_bytes = sizeof(PairDesc) * (size_t)np, task_bytes = sizeof(int2) * tasks.size();
-    int rc = descs.reserve(ctx, pair_bytes + task_bytes + 256);
-    if (rc) return rc;
-    std::vector<unsigned char> blob(pair_bytes + task_bytes);
-    std::memcpy(blob.data(), pairs.data(), pair_bytes);
-    if (task_bytes) std::memcpy(blob.data() + pair_bytes, tasks.data(), task_bytes);
-    if ((rc = upload(ctx, descs.ptr, blob.data(), blob.size(), stream))) return rc;
+    const size_t base = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(base + pair_bytes + task_bytes);
+    std::memcpy(blob.data() + base, pairs.data(), pair_bytes);
+    if (task_bytes) std::memcpy(blob.data() + base + pair_bytes, tasks.data(), task_bytes);
     ChainArgs &A = L.args;
-    A.pairs = (const PairDesc *)descs.ptr;
-    A.tasks = (const int2 *)((unsigned char *)descs.ptr + pair_bytes);
+    A.pairs = (const PairDesc *)(uintptr_t)base;
+    A.tasks = (const int2 *)(uintptr_t)(base + pair_bytes);
     A.ntasks = (int32_t)tasks.size();
     A.ticket = ctrl;
     A.abort_flag = ctrl + 1;
@@ -174,6 +192,28 @@ int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, u
     L.npairs = np;
     L.max_n = max_n;
     L.tile_m = tm;
+    return LCS_OK;
+}
+
+void rebase_launch(PreparedLaunch &L, const void *device_blob)
+{
+    const uintptr_t b = (uintptr_t)device_blob;
+    L.args.pairs = (const PairDesc *)(b + (uintptr_t)L.args.pairs);
+    L.args.tasks = (const int2 *)(b + (uintptr_t)L.args.tasks);
+}
+
+// Prepares one chain launch over `pairs` (one tile height): assigns each
+// pair its ring inside `ring`, lists the tasks and uploads descriptors and
+// tasks to `descs` through the context's staging.
+int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
+                  DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w)
+{
+    std::vector<unsigned char> blob;
+    int rc = build_launch(ctx, pairs, tm, wps, ring, ctrl, group_w, blob, L);
+    if (rc || pairs.empty()) return rc;
+    if ((rc = descs.reserve(ctx, blob.size() + 256))) return rc;
+    if ((rc = upload(ctx, descs.ptr, blob.data(), blob.size(), stream))) return rc;
+    rebase_launch(L, descs.ptr);
     return LCS_OK;
 }
 
@@ -206,9 +246,12 @@ int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, h
 
 // Strips a single pair is cut into when its bands alone would leave SIMDs
 // idle: the merge tree of the steady ant joins the strips (k = m each).
+// By default only pairs with m <= kDefaultStripMaxM are cut (merges of
+// larger cores run over the whole GPU and are chosen explicitly).
+constexpr int32_t kDefaultStripMaxM = 32768;
 int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 {
-    if (m > kMaxCoreK || n < 2 * 8192) return 1;
+    if (m > kDefaultStripMaxM || n < 2 * 8192) return 1;
     const long long simds = 4LL * ctx->num_cus;
     const long long bands = bands_of(m, n, tm);
     int s = 1;
@@ -382,6 +425,22 @@ int check_abort(lcs_ctx *ctx)
     return LCS_ETIMEDOUT;
 }
 
+// The schedule a single-pair call runs with (host strings give the cost
+// model the alphabet; device strings are taken as alphabet 4).
+static int single_schedule(lcs_ctx *ctx, const int32_t *a_host, int32_t m, const int32_t *b_host, int32_t n,
+                           const lcs_schedule *sched, lcs_schedule &out)
+{
+    if (sched && sched->tile_m) {
+        out = *sched;
+        return LCS_OK;
+    }
+    const Selector &S = ctx->select[LCS_TUNE_SINGLE];
+    int32_t alphabet = 4;
+    if ((S.model || !S.miss_cands.empty()) && (a_host || b_host))
+        alphabet = std::max(alphabet_of(a_host, a_host ? m : 0), alphabet_of(b_host, b_host ? n : 0));
+    return choose_schedule(ctx, LCS_TUNE_SINGLE, m, n, alphabet, 0.1f, sched, &out, nullptr);
+}
+
 }  // namespace lcs
 
 using namespace lcs;
@@ -412,7 +471,7 @@ int lcs_ctx_create(int device, size_t hbm_budget, lcs_ctx **out)
     ctx->budget = hbm_budget ? hbm_budget : free_b / 4 * 3;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return LCS_EIO; }
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess ||
+        ctx->staging.init() != LCS_OK ||
         hipEventCreateWithFlags(&ctx->ws_done, hipEventDisableTiming) != hipSuccess) {
         lcs_ctx_destroy(ctx);
         return LCS_EIO;
@@ -434,8 +493,7 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     ctx->ring.release(ctx);
     ctx->descs.release(ctx);
     ctx->strip_ws.release(ctx);
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
+    ctx->staging.destroy();
     if (ctx->ws_done) (void)hipEventDestroy(ctx->ws_done);
     ctx->in_a.release(ctx);
     ctx->in_b.release(ctx);
@@ -512,6 +570,9 @@ int lcs_comb(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_t *b, int32_
     if ((long long)m + n > 0x7fffffffLL) return LCS_ERANGE;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
     int rc;
+    lcs_schedule chosen{};
+    if ((rc = single_schedule(ctx, a, m, b, n, sched, chosen))) return rc;
+    sched = &chosen;
     if ((rc = ctx->in_a.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
     if ((rc = ctx->in_b.reserve(ctx, (size_t)std::max(n, 1) * 4))) return rc;
     if ((rc = ctx->out_h.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
@@ -546,6 +607,9 @@ int lcs_comb_permutation(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_
     if ((long long)m + n > 0x7fffffffLL) return LCS_ERANGE;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
     int rc;
+    lcs_schedule chosen{};
+    if ((rc = single_schedule(ctx, a, m, b, n, sched, chosen))) return rc;
+    sched = &chosen;
     if ((rc = ctx->in_a.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
     if ((rc = ctx->in_b.reserve(ctx, (size_t)std::max(n, 1) * 4))) return rc;
     if ((rc = ctx->out_h.reserve(ctx, (size_t)std::max(m, 1) * 4))) return rc;
@@ -586,8 +650,11 @@ int lcs_comb_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *
     if ((long long)m + n > 0x7fffffffLL) return LCS_ERANGE;
     if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    lcs_schedule chosen{};
+    int rc = single_schedule(ctx, nullptr, m, nullptr, n, sched, chosen);
+    if (rc) return rc;
     if (hipEventRecord(ctx->ev0, s) != hipSuccess) return LCS_EIO;
-    int rc = enqueue_chain(ctx, d_a, m, d_b, n, sched, d_h_out, d_v_out, s);
+    rc = enqueue_chain(ctx, d_a, m, d_b, n, &chosen, d_h_out, d_v_out, s);
     if (rc) return rc;
     if (hipEventRecord(ctx->ev1, s) != hipSuccess) return LCS_EIO;
     ctx->pending_timing = true;

@@ -45,6 +45,19 @@ int reserve_work(lcs_mlp *M, int N, float **X, float **T, float **Mk, float **A1
 }
 }  // namespace
 
+namespace lcs {
+const lcs_ctx *mlp_ctx(const lcs_mlp *M) { return M ? M->ctx : nullptr; }
+
+int mlp_shape(const lcs_mlp *M, int *n_in, int *hidden, int *n_out)
+{
+    if (!M) return LCS_EINVAL;
+    if (n_in) *n_in = M->S.I;
+    if (hidden) *hidden = M->S.H;
+    if (n_out) *n_out = M->S.O;
+    return LCS_OK;
+}
+}  // namespace lcs
+
 extern "C" {
 
 int lcs_mlp_create(lcs_ctx *ctx, int32_t n_in, int32_t hidden, int32_t n_out, uint64_t seed, lcs_mlp **out)
@@ -90,7 +103,7 @@ int lcs_mlp_create(lcs_ctx *ctx, int32_t n_in, int32_t hidden, int32_t n_out, ui
 int lcs_mlp_destroy(lcs_mlp *M)
 {
     if (!M) return LCS_EINVAL;
-    if (M->ctx && M->ctx->cost_model == M) M->ctx->cost_model = nullptr;
+    if (M->ctx) detach_model(M->ctx, M);   // a context the model was attached to forgets it
     M->params.release(M->ctx);
     M->adam_m.release(M->ctx);
     M->adam_v.release(M->ctx);

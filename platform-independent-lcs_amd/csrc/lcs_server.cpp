@@ -116,9 +116,21 @@ size_t footprint(int32_t m, int32_t n)
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    DevBuf descs[kClasses];
+    DevBuf descs;                  // every class's descriptors, one upload
+    Staging staging;               // this slot's own pinned upload buffer
     std::vector<std::shared_ptr<Request>> reqs;
     bool busy = false;
+};
+
+// A cost model handed over by lcs_server_set_cost_model: the parameters are
+// copied on the caller's thread and the worker builds its own model from
+// them on the server's private context, so the caller's context and model
+// are never touched by the worker.
+struct ModelUpdate {
+    bool pending = false;
+    int n_in = 0, hidden = 0, n_out = 0;
+    std::vector<float> params;     // empty: detach the model
+    std::vector<lcs_schedule> candidates;
 };
 
 }  // namespace
@@ -131,8 +143,11 @@ struct lcs_server {
     Arena arena;
     std::vector<Slot> slots;
     unsigned *ctrl = nullptr;       // 2 words per (slot, class)
+    // the worker's own copy of the cost model (on ctx) and its candidates;
+    // read and written by the worker thread only
     lcs_mlp *model = nullptr;
     std::vector<lcs_schedule> candidates;
+    ModelUpdate update;             // guarded by mu
 
     std::mutex mu;
     std::condition_variable cv;
@@ -193,19 +208,52 @@ int launch_slot(lcs_server *S, Slot &slot)
         P.E = E;  // prepare_pairs assigns rings from `ring`; each pair's own lies right here
         groups[(size_t)c].push_back(P);
     }
+    // every class's descriptors go up in one copy before any kernel is
+    // queued, so no upload waits behind a chain kernel of this slot
     const int slot_index = (int)(&slot - S->slots.data());
+    std::vector<unsigned char> blob;
+    PreparedLaunch launches[kClasses];
     for (int c = 0; c < kClasses; ++c) {
         auto &g = groups[(size_t)c];
         if (g.empty()) continue;
-        // each pair's ring lies inside its request's arena block
-        PreparedLaunch L;
-        int rc = prepare_pairs_own_rings(ctx, g, kTiles[c % kNumTiles], 1, slot.descs[c],
-                                         S->ctrl + 2 * (slot_index * kClasses + c), L, slot.stream,
-                                         c < kNumTiles ? 1 : 4);
+        // each pair's ring lies inside its request's arena block (ring = NULL)
+        const int rc = build_launch(ctx, g, kTiles[c % kNumTiles], 1, nullptr,
+                                    S->ctrl + 2 * (slot_index * kClasses + c), c < kNumTiles ? 1 : 4, blob,
+                                    launches[c]);
         if (rc) return rc;
-        if ((rc = launch_prepared(L, slot.stream))) return rc;
+    }
+    if (blob.empty()) return hipEventRecord(slot.done, slot.stream) == hipSuccess ? LCS_OK : LCS_EIO;
+    int rc = slot.descs.reserve(ctx, blob.size() + 256);
+    if (rc) return rc;
+    if ((rc = upload_via(slot.staging, slot.descs.ptr, blob.data(), blob.size(), slot.stream))) return rc;
+    for (int c = 0; c < kClasses; ++c) {
+        if (groups[(size_t)c].empty()) continue;
+        rebase_launch(launches[c], slot.descs.ptr);
+        if ((rc = launch_prepared(launches[c], slot.stream))) return rc;
     }
     return hipEventRecord(slot.done, slot.stream) == hipSuccess ? LCS_OK : LCS_EIO;
+}
+
+// Installs a cost model handed over by lcs_server_set_cost_model (worker
+// thread, the only user of the server's context).
+int apply_model_update(lcs_server *S, ModelUpdate &u)
+{
+    if (S->model) {
+        lcs_mlp_destroy(S->model);
+        S->model = nullptr;
+    }
+    S->candidates.clear();
+    if (u.params.empty()) return LCS_OK;
+    lcs_mlp *M = nullptr;
+    int rc = lcs_mlp_create(S->ctx, u.n_in, u.hidden, u.n_out, 0, &M);
+    if (rc) return rc;
+    if ((rc = lcs_mlp_set_params(M, u.params.data()))) {
+        lcs_mlp_destroy(M);
+        return rc;
+    }
+    S->model = M;
+    S->candidates = std::move(u.candidates);
+    return LCS_OK;
 }
 
 void finish_slot(lcs_server *S, Slot &slot)
@@ -255,6 +303,13 @@ void worker_main(lcs_server *S)
         if (free_slot) {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->stop && S->pending.empty() && busy == 0) break;
+            // a model set before a request was submitted decides that
+            // request's schedule: take the update with the admission
+            ModelUpdate u;
+            if (S->update.pending) {
+                u = std::move(S->update);
+                S->update = ModelUpdate{};
+            }
             while (!S->pending.empty()) {
                 auto R = S->pending.front();
                 if (R->cancelled) {
@@ -270,8 +325,15 @@ void worker_main(lcs_server *S)
                 free_slot->reqs.push_back(R);
                 S->pending.pop_front();
             }
+            lk.unlock();
+            if (u.pending) {
+                const int urc = apply_model_update(S, u);
+                if (urc) {
+                    std::lock_guard<std::mutex> g(S->mu);
+                    S->worker_error = urc;
+                }
+            }
             if (!free_slot->reqs.empty()) {
-                lk.unlock();
                 const int rc = launch_slot(S, *free_slot);
                 free_slot->busy = true;
                 launched = true;
@@ -334,7 +396,7 @@ int lcs_server_create(lcs_ctx *ctx, size_t hbm_budget, int32_t max_inflight, lcs
     S->slots.resize((size_t)max_inflight);
     for (auto &slot : S->slots)
         if (hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess || slot.staging.init() != LCS_OK) {
             lcs_server_destroy(S);
             return LCS_EIO;
         }
@@ -357,8 +419,10 @@ int lcs_server_destroy(lcs_server *S)
     for (auto &slot : S->slots) {
         if (slot.stream) { (void)hipStreamSynchronize(slot.stream); (void)hipStreamDestroy(slot.stream); }
         if (slot.done) (void)hipEventDestroy(slot.done);
-        for (auto &d : slot.descs) d.release(S->ctx);
+        slot.descs.release(S->ctx);
+        slot.staging.destroy();
     }
+    if (S->model) lcs_mlp_destroy(S->model);
     if (S->ctrl) (void)hipFree(S->ctrl);
     if (S->arena_base) (void)hipFree(S->arena_base);
     if (S->ctx) lcs_ctx_destroy(S->ctx);
@@ -373,9 +437,24 @@ int lcs_server_set_cost_model(lcs_server *S, lcs_mlp *model, const lcs_schedule 
         const int gw = group_width(candidates[i].variant);
         if (!gw || !candidates[i].tile_m || class_index(candidates[i].tile_m, gw) < 0) return LCS_EINVAL;
     }
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->model = model;
-    S->candidates.assign(candidates, candidates + ncand);
+    ModelUpdate u;
+    u.pending = true;
+    if (model && ncand > 0) {
+        // copy the parameters here, on the caller's thread (the model's
+        // context belongs to the caller); the worker rebuilds the model
+        int32_t count = 0;
+        int rc = lcs_mlp_param_count(model, &count);
+        if (!rc) rc = mlp_shape(model, &u.n_in, &u.hidden, &u.n_out);
+        if (rc) return rc;
+        u.params.resize((size_t)count);
+        if ((rc = lcs_mlp_get_params(model, u.params.data()))) return rc;
+        u.candidates.assign(candidates, candidates + ncand);
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->update = std::move(u);
+    }
+    S->cv.notify_all();
     return LCS_OK;
 }
 

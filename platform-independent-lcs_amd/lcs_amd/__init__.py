@@ -28,6 +28,8 @@ ERRORS = {
 }
 
 VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS = 0, 1, 2
+TUNE_SINGLE, TUNE_BATCH = 0, 1
+SCHED_FROM = {0: "default", 1: "table", 2: "model", 3: "autotune", 4: "caller"}
 SEMI_STRING_SUBSTRING, SEMI_SUBSTRING_STRING, SEMI_PREFIX_SUFFIX, SEMI_SUFFIX_PREFIX, SEMI_DOMINANCE = 0, 1, 2, 3, 4
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -120,6 +122,11 @@ def lib() -> ctypes.CDLL:
     L.lcs_server_request_times.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.lcs_server_stats.argtypes = [vp, _i64p, _i64p, _i64p, _i32p, ctypes.POINTER(sz)]
     L.lcs_server_set_cost_model.argtypes = [vp, vp, sched_p, i32]
+    L.lcs_ctx_add_tuned.argtypes = [vp, i32, i32, i32, sched_p]
+    L.lcs_ctx_clear_tuned.argtypes = [vp, i32]
+    L.lcs_ctx_set_cost_model.argtypes = [vp, i32, vp, sched_p, i32]
+    L.lcs_ctx_set_autotune_on_miss.argtypes = [vp, i32, sched_p, i32, i32]
+    L.lcs_ctx_choose_schedule.argtypes = [vp, i32, i32, i32, i32, ctypes.c_float, sched_p, _i32p]
     # workloads
     L.lcs_gen_binary_mt19937.argtypes = [i32, i32, _i32p]
     L.lcs_gen_uniform.argtypes = [ctypes.c_uint64, i32, i32, _i32p]
@@ -249,6 +256,31 @@ class Context:
         _check("lcs_autotune_measure", lib().lcs_autotune_measure(
             self._h, m, n, alphabet, rate, seed, arr, len(candidates), mode, reps, _f(out)))
         return out
+
+    # ---- schedule selection (table, model, autotune on a miss, default)
+    def add_tuned(self, mode: int, m: int, n: int, best: Schedule) -> None:
+        _check("lcs_ctx_add_tuned", lib().lcs_ctx_add_tuned(self._h, mode, m, n, ctypes.byref(best)))
+
+    def clear_tuned(self, mode: int) -> None:
+        _check("lcs_ctx_clear_tuned", lib().lcs_ctx_clear_tuned(self._h, mode))
+
+    def set_cost_model(self, mode: int, mlp, candidates) -> None:
+        """Attach (mlp, one output per candidate) for `mode`; mlp None detaches."""
+        arr = (Schedule * max(len(candidates), 1))(*candidates) if candidates else None
+        _check("lcs_ctx_set_cost_model", lib().lcs_ctx_set_cost_model(
+            self._h, mode, mlp._h if mlp is not None else None, arr, len(candidates) if mlp is not None else 0))
+
+    def set_autotune_on_miss(self, mode: int, candidates, reps: int = 1) -> None:
+        arr = (Schedule * max(len(candidates), 1))(*candidates) if candidates else None
+        _check("lcs_ctx_set_autotune_on_miss", lib().lcs_ctx_set_autotune_on_miss(
+            self._h, mode, arr, len(candidates), reps))
+
+    def choose_schedule(self, mode: int, m: int, n: int, alphabet: int = 4, rate: float = 0.1):
+        """(Schedule, source) the context would use for a pair of this shape."""
+        out, src = Schedule(), ctypes.c_int32()
+        _check("lcs_ctx_choose_schedule", lib().lcs_ctx_choose_schedule(
+            self._h, mode, m, n, alphabet, rate, ctypes.byref(out), ctypes.byref(src)))
+        return out, SCHED_FROM[src.value]
 
     def string_substring(self, row_to_col, m: int, n: int, qi, qj) -> np.ndarray:
         """LCS(a, b[qi:qj)) per query, read from P(a, b) on the GPU."""

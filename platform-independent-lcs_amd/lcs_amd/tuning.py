@@ -114,6 +114,38 @@ def load_table(mode: int = 1) -> dict:
         return json.load(f)
 
 
+def best_rows(table: dict):
+    """(m, n, best Schedule) of every measured shape of a table."""
+    cands = [Schedule(*c) for c in table["candidates"]]
+    out = []
+    for r in table["rows"]:
+        c = np.array(r["cost_ms"], np.float64)
+        c = np.where(np.isfinite(c) & (c > 0), c, np.inf)
+        if np.isfinite(c).any():
+            out.append((int(r["m"]), int(r["n"]), cands[int(np.argmin(c))]))
+    return out
+
+
+def load_into(ctx: Context, modes=(0, 1), model: bool = False, steps: int = 4000):
+    """Loads the committed autotune tables of this device into the context's
+    schedule selection (step 1 of lcs_ctx_choose_schedule); with model=True
+    also trains a cost model per mode on the GPU and attaches it (step 2).
+    Returns the attached models (close them after the context stops using
+    them)."""
+    models = {}
+    for mode in modes:
+        if not os.path.exists(TABLES[mode]):
+            continue
+        table = load_table(mode)
+        for m, n, s in best_rows(table):
+            ctx.add_tuned(mode, m, n, s)
+        if model:
+            mlp = train(ctx, table, steps=steps)
+            ctx.set_cost_model(mode, mlp, [Schedule(*c) for c in table["candidates"]])
+            models[mode] = mlp
+    return models
+
+
 def regret(C: np.ndarray, choice: np.ndarray) -> np.ndarray:
     """Cost of the chosen schedule over the best measured one, per row."""
     best = np.min(np.where(np.isfinite(C), C, np.inf), axis=1)

@@ -51,3 +51,35 @@ def test_budget_admission_and_errors(ctx):
     assert all(np.array_equal(h, h1) and np.array_equal(v, v1) for h, v in outs)
     assert lcs_amd.lib().lcs_server_cancel(srv._h, 10**9) == -2   # unknown request id: LCS_ENOENT
     srv.close()
+
+
+def test_cost_model_swapped_while_requests_run(ctx):
+    """The model handed to the server is copied: the worker builds its own
+    on the server's context, so the caller may detach it, attach another
+    or destroy its own while requests run, and every result stays equal to
+    the pair combed alone."""
+    from lcs_amd import tuning
+    mlp = lcs_amd.Mlp(ctx, n_in=4, hidden=16, n_out=len(tuning.CANDIDATES), seed=3)
+    pairs = lcs_amd.workload_batch(40, seed=23, lo=512, hi=12000, alphabet=(4, 20))
+    feats = tuning.features([p.a.size for p in pairs], [p.b.size for p in pairs], [p.alphabet for p in pairs],
+                            [p.rate for p in pairs])
+    srv = lcs_amd.Server(ctx, hbm_budget=64 << 20, max_inflight=3)
+    try:
+        srv.set_cost_model(mlp, tuning.CANDIDATES)
+        ids = []
+        for i, (p, f) in enumerate(zip(pairs, feats)):
+            ids.append(srv.submit(p.a, p.b, None, f))
+            if i == 10:
+                srv.set_cost_model(None, [])               # detach mid-stream
+            if i == 20:
+                srv.set_cost_model(mlp, tuning.CANDIDATES)  # attach again
+            if i == 30:
+                mlp.close()                                 # the caller's model goes away
+        outs = [srv.wait(i, timeout_ms=60000) for i in ids]
+        assert srv.stats()["status"] == 0
+    finally:
+        srv.close()
+        mlp.close()
+    for p, (h, v) in zip(pairs, outs):
+        h1, v1, _ = ctx.comb(p.a, p.b)
+        assert np.array_equal(h, h1) and np.array_equal(v, v1)
