@@ -3,15 +3,18 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Largest padded core (2^L) the workgroup kernel holds in LDS.
-constexpr int kMaxCoreK = 32768;
+// Largest padded core (2^L): cores up to 2^14 run in one workgroup's LDS,
+// larger ones over the whole GPU (their grid of ant-walk boxes is sized
+// for nodes of at most 2^22 points).
+constexpr int kMaxCoreL = 22;
+constexpr int kMaxCoreK = 1 << kMaxCoreL;
 
 struct BraidCoreDesc {
     const int32_t *P;   // k: row -> shared column
     const int32_t *Q;   // k: shared row -> column
     int32_t *R;         // k: row -> column of the sticky product
     int32_t k, L;       // padded size 2^L >= k
-    int32_t *scratch;   // lcsb_core_scratch_words(k) words
+    int32_t *scratch;   // lcsb_core_scratch_words(k) words (k > 2^14: the big-core workspace)
 };
 
 // Core descriptors passed by value in the kernel arguments, one per block.

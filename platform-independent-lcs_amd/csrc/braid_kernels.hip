@@ -61,51 +61,6 @@ __device__ __forceinline__ int wave_incl_scan(int x)
     return x;
 }
 
-// Exclusive prefix of the 0/1 flags of keys[0..K) (bit `bit`), written to
-// S[0..K] (S[K] = total).  Each thread owns a contiguous run of elements.
-__device__ void block_scan_bits(const int32_t *keys, int K, int bit, int32_t *S, int32_t *lds_sums)
-{
-    const int t = threadIdx.x;
-    const int per = (K + kThreads - 1) / kThreads;
-    const int lo = min(t * per, K), hi = min(lo + per, K);
-    int cnt = 0;
-    for (int x = lo; x < hi; ++x) cnt += (keys[x] >> bit) & 1;
-    lds_sums[t] = cnt;
-    __syncthreads();
-    // Hillis-Steele over the 1024 per-thread sums
-    for (int off = 1; off < kThreads; off <<= 1) {
-        int v = t >= off ? lds_sums[t - off] : 0;
-        __syncthreads();
-        lds_sums[t] += v;
-        __syncthreads();
-    }
-    int run = lds_sums[t] - cnt;  // exclusive
-    for (int x = lo; x < hi; ++x) {
-        S[x] = run;
-        run += (keys[x] >> bit) & 1;
-    }
-    if (t == kThreads - 1) S[K] = lds_sums[kThreads - 1];
-    __syncthreads();
-}
-
-// One depth of the top-down split: stable partition of every aligned segment
-// of size s by bit `bit` of the key (zeros first).  par[pos] = old position.
-__device__ void split_depth(const int32_t *key, int32_t *key_new, int32_t *par, int K, int s, int bit, int32_t *S,
-                            int32_t *lds_sums)
-{
-    block_scan_bits(key, K, bit, S, lds_sums);
-    for (int x = threadIdx.x; x < K; x += kThreads) {
-        const int seg = x & ~(s - 1);
-        const int b = (key[x] >> bit) & 1;
-        const int ones_before = S[x] - S[seg];
-        const int zeros_total = s - (S[seg + s] - S[seg]);
-        const int pos = b ? seg + zeros_total + ones_before : seg + (x - seg - ones_before);
-        key_new[pos] = key[x];
-        par[pos] = x;
-    }
-    __syncthreads();
-}
-
 // One bottom-up depth (node size s) of the core: lift the children's
 // products into node coordinates (the reference's inverse_mapping, :191-202)
 // and run the chunked ant passage (:123-189) for every node of the depth.
@@ -334,49 +289,6 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     __syncthreads();
 }
 
-// The sticky product of the k-permutations P (row -> shared column) and Q
-// (shared row -> column) for every descriptor; one workgroup per product.
-// Keys and products in global memory: cores up to K = 32768.
-__global__ __launch_bounds__(kThreads) void braid_core_kernel(CoreBatch batch)
-{
-    const BraidCoreDesc D = batch.d[blockIdx.x];
-    const int k = D.k, L = D.L, K = 1 << L;
-    extern __shared__ uint16_t lds[];
-    uint16_t *U = lds;            // node-local row -> node-local col | lo flag
-    uint16_t *CI = lds + K;       // node-local col -> node-local row | lo flag
-    __shared__ int32_t sums[kThreads];
-    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
-    int32_t *keyP = D.scratch, *keyQ = keyP + K, *keyP2 = keyQ + K, *keyQ2 = keyP2 + K;
-    int32_t *S = keyQ2 + K;               // K+1
-    int32_t *R = S + K + 1, *Rn = R + K;  // per depth products
-    int32_t *bT = Rn + K, *bD = bT + K;   // per chunk entry column / delta
-    int32_t *parP = bD + K;               // L * K
-    int32_t *parQ = parP + (size_t)L * K; // L * K
-
-    // depth 0: P ordered by row (key = shared column), Q by column (key = shared row)
-    for (int x = threadIdx.x; x < K; x += kThreads) {
-        keyP[x] = x < k ? D.P[x] : x;
-        if (x >= k) keyQ[x] = x;
-    }
-    for (int r = threadIdx.x; r < k; r += kThreads)
-        if ((uint32_t)D.Q[r] < (uint32_t)k) keyQ[D.Q[r]] = r;
-    __syncthreads();
-    for (int d = 0; d < L; ++d) {
-        const int s = K >> d, bit = L - d - 1;
-        split_depth(keyP, keyP2, parP + (size_t)d * K, K, s, bit, S, sums);
-        split_depth(keyQ, keyQ2, parQ + (size_t)d * K, K, s, bit, S, sums);
-        int32_t *t0 = keyP; keyP = keyP2; keyP2 = t0;
-        int32_t *t1 = keyQ; keyQ = keyQ2; keyQ2 = t1;
-    }
-    for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
-    __syncthreads();
-    for (int d = L - 1; d >= 0; --d) {
-        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, hist);
-        int32_t *tmp = R; R = Rn; Rn = tmp;
-    }
-    for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
-}
-
 // One depth of the top-down split with the keys in LDS: a stable partition
 // of every aligned segment of size s by bit `bit` (zeros first), the ranks
 // from wave ballots and one scan of the 64-key group counts.
@@ -463,25 +375,26 @@ __device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t 
     __syncthreads();
 }
 
-// The same product with keys, both products and the chunk entries in LDS:
-// cores up to K = kLdsCoreK.  LDS: 4K uint16 (keys / U, CI, R, Rn) plus
-// K/64+1 group counts.
-__global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batch)
+// The sticky product with keys, both products and the chunk entries in LDS
+// for cores up to K = kLdsCoreK, one workgroup per core.  LDS: 4K uint16
+// (keys / U, CI, R, Rn) plus K/64+1 group counts.  P holds row -> shared
+// column; Q holds shared row -> column, or (QKEY) column -> shared row,
+// the keys themselves.  Values are read modulo K (a leaf of a big core
+// reads its slice of the big core's keys).
+template <bool QKEY>
+__device__ __forceinline__ void core_lds_body(const int32_t *P, const int32_t *Q, int32_t *Rout, int k, int L,
+                                              int32_t *scratch, uint16_t *lds, int32_t *bT, int32_t *bD, int *hist)
 {
-    const BraidCoreDesc D = batch.d[blockIdx.x];
-    const int k = D.k, L = D.L, K = 1 << L;
-    extern __shared__ uint16_t lds[];
-    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
-    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
+    const int K = 1 << L, mask = K - 1;
     uint16_t *key = lds, *key2 = lds + K;             // top-down (P, then Q)
     uint16_t *U = lds, *CI = lds + K;                 // bottom-up (reuses the key space)
     uint16_t *R = lds + 2 * (size_t)K, *Rn = lds + 3 * (size_t)K;
     int32_t *gpre = (int32_t *)(lds + 4 * (size_t)K);
-    int32_t *parP = D.scratch;                        // L * K
+    int32_t *parP = scratch;                          // L * K
     int32_t *parQ = parP + (size_t)L * K;             // L * K
 
     // P ordered by row (key = shared column)
-    for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(x < k ? D.P[x] : x);
+    for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(x < k ? (P[x] & mask) : x);
     __syncthreads();
     for (int d = 0; d < L; ++d) {
         split_depth_lds(key, key2, parP + (size_t)d * K, K, K >> d, L - d - 1, gpre);
@@ -490,10 +403,14 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
     // Q ordered by column (key = shared row)
     key = lds;
     key2 = lds + K;
-    for (int x = threadIdx.x; x < K; x += kThreads)
-        if (x >= k) key[x] = (uint16_t)x;
-    for (int r = threadIdx.x; r < k; r += kThreads)
-        if ((uint32_t)D.Q[r] < (uint32_t)k) key[D.Q[r]] = (uint16_t)r;
+    if (QKEY) {
+        for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(Q[x] & mask);
+    } else {
+        for (int x = threadIdx.x; x < K; x += kThreads)
+            if (x >= k) key[x] = (uint16_t)x;
+        for (int r = threadIdx.x; r < k; r += kThreads)
+            if ((uint32_t)Q[r] < (uint32_t)k) key[Q[r]] = (uint16_t)r;
+    }
     __syncthreads();
     for (int d = 0; d < L; ++d) {
         split_depth_lds(key, key2, parQ + (size_t)d * K, K, K >> d, L - d - 1, gpre);
@@ -505,7 +422,463 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
         bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, hist);
         uint16_t *tmp = R; R = Rn; Rn = tmp;
     }
-    for (int x = threadIdx.x; x < k; x += kThreads) D.R[x] = R[x];
+    for (int x = threadIdx.x; x < k; x += kThreads) Rout[x] = R[x];
+}
+
+__global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batch)
+{
+    const BraidCoreDesc D = batch.d[blockIdx.x];
+    extern __shared__ uint16_t lds[];
+    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
+    core_lds_body<false>(D.P, D.Q, D.R, D.k, D.L, D.scratch, lds, bT, bD, hist);
+}
+
+// ---------------------------------------------------------------- big cores
+// Cores above kLdsCoreK run the same recursion spread over the whole GPU:
+//  * the top d0 = L - L0 depths of the top-down split are device-wide
+//    stable partitions (count, scan, scatter per depth);
+//  * the 2^d0 nodes of depth d0 are independent sticky products of 2^L0
+//    points each, one workgroup per node (core_lds_body on their keys);
+//  * bottom-up, each top depth lifts the children's products into node
+//    coordinates (inverse_mapping, src/braid_multiplication.hpp:191-202)
+//    and runs the ant passage (:123-189) cut into pieces that a wave each
+//    walks in LDS.  The ant leaves row r at T(r) = min{c : delta(r, c) >= 0}
+//    (delta non-decreasing in r and c), so its path is fixed by where it
+//    crosses a grid of lines every g rows and every g columns of a node:
+//      - row line Dr (= s - j g): enters row Dr-1 at column T(Dr), with
+//        delta(Dr, T) = 0;
+//      - column line c = i g: the ant steps right from (rho, c) to
+//        (rho, c+1) at rho + 1 = min{r : delta(r, c) >= 0}, with
+//        delta(rho, c) = -1.
+//    Both are "first position where a running count reaches a target" over
+//    the node's columns (row lines) or rows (column lines): with bucket
+//    b(e) = (s-1-idx(e)) >> log2 g of an entry and boundary j,
+//        z_j(e) = lo(e) ? b(e) < j : b(e) >= j,  Y_j = #{lo entries: b < j},
+//        T_j = min{x : #{x' < x : z_j(e_x')} >= Y_j},
+//    the column line c = i g being boundary j = s/g - i over the rows
+//    (T_j = rho + 1).  Segment histograms, a scan over the segments per
+//    boundary and a sweep of the one segment where the count crosses find
+//    every T_j.  A piece starts at a crossing and ends at the next one, so
+//    it stays inside one g x g box and takes at most 2g + 1 steps.
+// Entries of U (row -> column) and CI (column -> row) are node-local
+// indices with the lo flag in bit 31.
+constexpr int kBigBatch = 16;          // products per big-core launch
+constexpr int kSplitTile = 2048;       // elements per block of a device-wide partition
+constexpr int kHistSeg = 8192;         // entries per block of the boundary histograms
+constexpr int kMaxLines = 8192;        // grid lines per node and direction
+constexpr uint32_t kLo31 = 0x80000000u;
+
+struct BigCore {
+    const int32_t *P, *Q;   // k: row -> shared column, shared row -> column
+    int32_t *R;             // k: row -> column of the product
+    int32_t *ws;            // lcsb_big_ws_words(k, L0) words
+    int32_t k;
+};
+struct BigBatch {
+    BigCore c[kBigBatch];
+    int32_t L, L0;
+};
+
+// grid of one top depth (node size s): lines every g = s / nb
+struct BigDepth {
+    int32_t s, nb, glog, w, nsegs;   // w entries per histogram block, nsegs per node
+};
+
+__host__ __device__ inline BigDepth big_depth(int L, int d)
+{
+    BigDepth D;
+    D.s = 1 << (L - d);
+    D.nb = D.s / 64 < kMaxLines ? D.s / 64 : kMaxLines;
+    D.glog = __builtin_ctz((unsigned)(D.s / D.nb));
+    D.w = D.s < kHistSeg ? D.s : kHistSeg;
+    D.nsegs = D.s / D.w;
+    return D;
+}
+
+// word offsets inside one product's workspace
+struct BigLayout {
+    size_t keyP, keyQ, keyP2, keyQ2, parP, parQ, leaf, R, Rn, U, CI, cnt, segtot, hist, info, T, total;
+};
+
+__host__ __device__ inline BigLayout big_layout(int L, int L0)
+{
+    const size_t K = (size_t)1 << L;
+    const int d0 = L - L0;
+    BigLayout B;
+    size_t o = 0;
+    B.keyP = o; o += K;
+    B.keyQ = o; o += K;
+    B.keyP2 = o; o += K;
+    B.keyQ2 = o; o += K;
+    B.parP = o; o += (size_t)d0 * K;
+    B.parQ = o; o += (size_t)d0 * K;
+    B.leaf = o; o += (size_t)2 * L0 * K;     // per leaf: L0 * 2^L0 parents of P and of Q
+    B.R = o; o += K;
+    B.Rn = o; o += K;
+    B.U = o; o += K;
+    B.CI = o; o += K;
+    B.cnt = o; o += 2 * (K / kSplitTile);
+    B.segtot = o; o += 2 * ((size_t)1 << d0);
+    size_t hist = 0, lines = 0;
+    for (int d = 0; d < d0; ++d) {
+        const BigDepth D = big_depth(L, d);
+        const size_t nodes = K / (size_t)D.s;
+        const size_t h = 2 * nodes * (size_t)D.nsegs * (size_t)(D.nb + 1);
+        hist = h > hist ? h : hist;
+        lines = nodes * (size_t)D.nb > lines ? nodes * (size_t)D.nb : lines;
+    }
+    B.hist = o; o += hist;
+    B.info = o; o += 2 * 2 * lines;          // (segment, rank) per boundary
+    B.T = o; o += 2 * lines;
+    B.total = o;
+    return B;
+}
+
+__device__ __forceinline__ uint32_t counted31(uint32_t e, int r) { return ((e - (uint32_t)r) >> 31) & 1u; }
+
+__global__ void big_init_kernel(BigBatch B)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int K = 1 << B.L, x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= K) return;
+    int32_t *keyP = C.ws + Y.keyP, *keyQ = C.ws + Y.keyQ;
+    keyP[x] = x < C.k ? C.P[x] : x;
+    if (x >= C.k) keyQ[x] = x;
+    else if ((uint32_t)C.Q[x] < (uint32_t)C.k) keyQ[C.Q[x]] = x;
+}
+
+// ones of bit `bit` per tile of kSplitTile keys (y = 0: P keys, 1: Q keys)
+__global__ __launch_bounds__(256) void big_split_count_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int K = 1 << B.L, bit = B.L - d - 1, y = blockIdx.y;
+    const int32_t *key = C.ws + (y ? ((d & 1) ? Y.keyQ2 : Y.keyQ) : ((d & 1) ? Y.keyP2 : Y.keyP));
+    const int base = blockIdx.x * kSplitTile + threadIdx.x * 8;
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c += (key[base + q] >> bit) & 1;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    __shared__ int red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) C.ws[Y.cnt + (size_t)y * (K / kSplitTile) + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// exclusive scan of the tile counts of segment blockIdx.x (size s) in place;
+// the segment's total ones to segtot
+__global__ __launch_bounds__(256) void big_split_scan_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int K = 1 << B.L, s = K >> d, y = blockIdx.y;
+    const int bps = s / kSplitTile;
+    int32_t *cnt = C.ws + Y.cnt + (size_t)y * (K / kSplitTile) + (size_t)blockIdx.x * bps;
+    const int per = (bps + 255) / 256;
+    const int lo = min((int)threadIdx.x * per, bps), hi = min(lo + per, bps);
+    int run = 0;
+    for (int i = lo; i < hi; ++i) run += cnt[i];
+    __shared__ int sums[256];
+    sums[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int acc = sums[threadIdx.x] - run;
+    for (int i = lo; i < hi; ++i) {
+        const int c = cnt[i];
+        cnt[i] = acc;
+        acc += c;
+    }
+    if (threadIdx.x == 255) C.ws[Y.segtot + (size_t)y * ((size_t)1 << (B.L - B.L0)) + blockIdx.x] = sums[255];
+}
+
+// stable partition of every segment of size s by bit `bit` (zeros first);
+// par[pos] = old position
+__global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int K = 1 << B.L, s = K >> d, bit = B.L - d - 1, y = blockIdx.y;
+    const int32_t *key = C.ws + (y ? ((d & 1) ? Y.keyQ2 : Y.keyQ) : ((d & 1) ? Y.keyP2 : Y.keyP));
+    int32_t *key_new = C.ws + (y ? ((d & 1) ? Y.keyQ : Y.keyQ2) : ((d & 1) ? Y.keyP : Y.keyP2));
+    int32_t *par = C.ws + (y ? Y.parQ : Y.parP) + (size_t)d * K;
+    const int base = blockIdx.x * kSplitTile + threadIdx.x * 8;
+    int32_t kv[8];
+    int c = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        kv[q] = key[base + q];
+        c += (kv[q] >> bit) & 1;
+    }
+    // exclusive prefix of the thread counts over the block
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int incl = wave_incl_scan(c);
+    __shared__ int wsum[4];
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int before = incl - c;
+    for (int w2 = 0; w2 < wave; ++w2) before += wsum[w2];
+    const int seg = (blockIdx.x * kSplitTile) & ~(s - 1);
+    const int segi = seg / s;
+    before += C.ws[Y.cnt + (size_t)y * (K / kSplitTile) + blockIdx.x];
+    const int ones_total = C.ws[Y.segtot + (size_t)y * ((size_t)1 << (B.L - B.L0)) + segi];
+    const int zeros_total = s - ones_total;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int x = base + q;
+        const int b = (kv[q] >> bit) & 1;
+        const int pos = b ? seg + zeros_total + before : seg + (x - seg - before);
+        before += b;
+        key_new[pos] = kv[q];
+        par[pos] = x;
+    }
+}
+
+// the 2^d0 independent nodes of depth d0, one workgroup each
+__global__ __launch_bounds__(kThreads) void big_leaf_kernel(BigBatch B)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int d0 = B.L - B.L0, s0 = 1 << B.L0;
+    const size_t v0 = (size_t)blockIdx.x * s0;
+    const int32_t *keyP = C.ws + ((d0 & 1) ? Y.keyP2 : Y.keyP) + v0;
+    const int32_t *keyQ = C.ws + ((d0 & 1) ? Y.keyQ2 : Y.keyQ) + v0;
+    extern __shared__ uint16_t lds[];
+    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
+    // depth d0 of the big core's own recursion: the products go to R
+    core_lds_body<true>(keyP, keyQ, C.ws + Y.R + v0, s0, B.L0, C.ws + Y.leaf + (size_t)blockIdx.x * 2 * B.L0 * s0,
+                        lds, bT, bD, hist);
+}
+
+// lift the children's products into node coordinates (node size s).  The
+// children's products are in R (flip = 0) or Rn (flip = 1), the node's go
+// to the other one.
+__global__ void big_lift_kernel(BigBatch B, int d, int flip)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int K = 1 << B.L, s = K >> d, h = s >> 1;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= K) return;
+    const int32_t *pp = C.ws + Y.parP + (size_t)d * K, *pq = C.ws + Y.parQ + (size_t)d * K;
+    const int32_t *R = C.ws + (flip ? Y.Rn : Y.R);
+    int32_t *Rn = C.ws + (flip ? Y.R : Y.Rn);
+    uint32_t *U = (uint32_t *)(C.ws + Y.U), *CI = (uint32_t *)(C.ws + Y.CI);
+    const int p_pos = pp[x];
+    const int cseg = x & ~(h - 1);
+    const int q_pos = pq[cseg + R[x]];
+    const int v0 = p_pos & ~(s - 1);
+    const uint32_t lo = (x & h) ? 0u : kLo31;
+    const int lr = p_pos - v0, lc = q_pos - v0;
+    U[p_pos] = (uint32_t)lc | lo;
+    CI[v0 + lc] = (uint32_t)lr | lo;
+    Rn[p_pos] = lc;
+}
+
+// Per histogram block (w entries of CI for y = 0, of U for y = 1): packed
+// exclusive prefix over the buckets of (lo count | hi count << 16) for each
+// boundary j < nb, and the packed totals at [nb].
+__global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigDepth D = big_depth(B.L, d);
+    const int K = 1 << B.L, y = blockIdx.y;
+    const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI));
+    __shared__ uint32_t h[kMaxLines];
+    __shared__ uint32_t sums[256];
+    for (int j = threadIdx.x; j < D.nb; j += 256) h[j] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * D.w;
+    for (int i = threadIdx.x; i < D.w; i += 256) {
+        const uint32_t e = E[base + i];
+        const int bk = (D.s - 1 - (int)(e & ~kLo31)) >> D.glog;
+        atomicAdd(&h[bk], (e & kLo31) ? 1u : 0x10000u);
+    }
+    __syncthreads();
+    const int per = D.nb / 256 > 0 ? D.nb / 256 : 1;   // nb is a power of two >= 64
+    const int lo = min((int)threadIdx.x * per, D.nb), hi = min(lo + per, D.nb);
+    uint32_t run = 0;
+    for (int j = lo; j < hi; ++j) run += h[j];
+    sums[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sums[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t acc = sums[threadIdx.x] - run;
+    uint32_t *out = (uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + blockIdx.x) * (size_t)(D.nb + 1);
+    for (int j = lo; j < hi; ++j) {
+        out[j] = acc;
+        acc += h[j];
+    }
+    if (threadIdx.x == 255) out[D.nb] = sums[255];
+}
+
+// One thread per (y, node, boundary j): Y_j from the blocks' lo prefixes,
+// then the block where the running count reaches Y_j and the rank still
+// needed inside it.  Boundaries with Y_j = 0 (or j = 0) get T = 0 here.
+__global__ __launch_bounds__(256) void big_bound_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigDepth D = big_depth(B.L, d);
+    const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes * D.nb) return;
+    const int node = t / D.nb, j = t % D.nb;
+    const size_t stride = (size_t)D.nb + 1;
+    const uint32_t *H = (const uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + (size_t)node * D.nsegs) * stride;
+    int32_t *info = C.ws + Y.info + 2 * ((size_t)y * nodes * D.nb + t);
+    int32_t *T = C.ws + Y.T + (size_t)y * nodes * D.nb + t;
+    int Yj = 0;
+    if (j > 0)
+        for (int g = 0; g < D.nsegs; ++g) Yj += (int)(H[g * stride + j] & 0xffffu);
+    if (j == 0 || Yj == 0) {
+        info[0] = -1;
+        info[1] = 0;
+        *T = 0;
+        return;
+    }
+    int Z = 0, seg = D.nsegs - 1;
+    for (int g = 0; g < D.nsegs; ++g) {
+        const uint32_t P = H[g * stride + j], tot = H[g * stride + D.nb];
+        const int c = (int)(P & 0xffffu) + (int)(tot >> 16) - (int)(P >> 16);
+        if (Z + c >= Yj) {
+            seg = g;
+            break;
+        }
+        Z += c;
+    }
+    info[0] = seg;
+    info[1] = Yj - Z;
+}
+
+// One wave per (y, node, boundary) left open: sweep its block for the
+// entry where the count reaches the rank; T = that entry's position + 1.
+__global__ __launch_bounds__(256) void big_sweep_kernel(BigBatch B, int d)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigDepth D = big_depth(B.L, d);
+    const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= nodes * D.nb) return;
+    const int32_t *info = C.ws + Y.info + 2 * ((size_t)y * nodes * D.nb + t);
+    const int seg = info[0];
+    if (seg < 0) return;
+    int need = info[1];
+    const int node = t / D.nb, j = t % D.nb;
+    const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI)) + (size_t)node * D.s + (size_t)seg * D.w;
+    const unsigned long long lt = (1ull << lane) - 1;
+    for (int r0 = 0; r0 < D.w; r0 += 64) {
+        const uint32_t e = E[r0 + lane];
+        const int bk = (D.s - 1 - (int)(e & ~kLo31)) >> D.glog;
+        const bool z = (e & kLo31) ? bk < j : bk >= j;
+        const unsigned long long ball = __ballot(z);
+        const int cnt = __popcll(ball);
+        if (need <= cnt) {
+            const bool hit = z && __popcll(ball & lt) == need - 1;
+            const int pos = __ffsll((long long)__ballot(hit)) - 1;
+            if (lane == 0) C.ws[Y.T + (size_t)y * nodes * D.nb + t] = seg * D.w + r0 + pos + 1;
+            return;
+        }
+        need -= cnt;
+    }
+}
+
+// One wave per piece of the ant's path: the piece's g x g box (plus the
+// row above and the column to its left) is staged in LDS and lane 0 walks
+// it with the reference's incremental updates, placing the good points.
+// Pieces of a node: q < nb starts at row line j = q (q = 0: the start of
+// the passage at (s-1, 0)), q >= nb at column line i = q - nb + 1.
+constexpr int kMaxBoxG = 512;   // g at the largest node (2^22 / kMaxLines)
+__global__ __launch_bounds__(256) void big_walk_kernel(BigBatch B, int d, int flip)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigDepth D = big_depth(B.L, d);
+    const int K = 1 << B.L, s = D.s, nodes = K / s, g = 1 << D.glog, nb = D.nb;
+    const int pieces = 2 * nb - 1;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (t >= nodes * pieces) return;
+    const int node = t / pieces, q = t % pieces;
+    const size_t v0 = (size_t)node * s;
+    const uint32_t *U = (const uint32_t *)(C.ws + Y.U) + v0, *CI = (const uint32_t *)(C.ws + Y.CI) + v0;
+    int32_t *Rn = C.ws + (flip ? Y.R : Y.Rn) + v0;
+    const int32_t *Trow = C.ws + Y.T + (size_t)node * nb;
+    const int32_t *Tcol = C.ws + Y.T + (size_t)nodes * nb + (size_t)node * nb;
+    int dr, dc, delta, r_lo, cb;
+    if (q < nb) {
+        const int Dr = s - q * g;
+        dc = q == 0 ? 0 : Trow[q];
+        dr = Dr - 1;
+        delta = -(int)counted31(U[dr], dc);
+        r_lo = Dr - g;
+        cb = dc == 0 ? 0 : (dc - 1) >> D.glog;
+    } else {
+        const int i = q - nb + 1;
+        const int Tc = Tcol[nb - i];
+        if (Tc == 0) return;   // the ant never reaches this column line
+        dr = Tc - 1;
+        dc = i * g + 1;
+        delta = -1 + (int)counted31(CI[i * g], dr);
+        r_lo = s - (((s - 1 - dr) >> D.glog) + 1) * g;
+        cb = i;
+    }
+    const int r_hi = r_lo + g;
+    const int c_lo = cb ? cb * g + 1 : 0, c_hi = cb == nb - 1 ? s + 1 : (cb + 1) * g + 1;
+    // box: rows r_lo-1 .. r_hi-1, columns c_lo-1 .. c_hi-1 (clamped into the node)
+    __shared__ uint32_t boxU[4][kMaxBoxG + 1], boxC[4][kMaxBoxG + 2];
+    uint32_t *bu = boxU[wave], *bc = boxC[wave];
+    const int nu = g + 1, nc = c_hi - c_lo + 1;
+    for (int i = lane; i < nu; i += 64) bu[i] = U[min(max(r_lo - 1 + i, 0), s - 1)];
+    for (int i = lane; i < nc; i += 64) bc[i] = CI[min(max(c_lo - 1 + i, 0), s - 1)];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+    // U(r) = bu[r - r_lo + 1], CI(c) = bc[c - c_lo + 1]; reads clamped like the node-level walk
+    auto Ur = [&](int r) { return bu[min(max(r, r_lo - 1), r_hi - 1) - r_lo + 1]; };
+    auto Cc = [&](int c) { return bc[min(max(c, c_lo - 1), c_hi - 1) - c_lo + 1]; };
+    uint32_t cl = Cc(max(dc - 1, 0)), cc = Cc(min(dc, s - 1));
+    uint32_t ud = Ur(dr), ur = Ur(max(dr - 1, 0));
+    while (dr >= r_lo) {
+        const int d_left = delta - (int)counted31(cl, dr);
+        const int d_down = delta + (int)counted31(ud, dc);
+        if (dc > 0 && d_left < 0 && d_down > 0) Rn[dr] = dc - 1;
+        const bool right = delta < 0 && dc < s;
+        if (right) {
+            delta += (int)counted31(cc, dr);
+            ++dc;
+            if (dc == c_hi) break;   // the next piece starts at this column line
+            cl = cc;
+            cc = Cc(min(dc, s - 1));
+        } else {
+            delta -= dr > 0 ? (int)counted31(ur, dc) : 0;
+            --dr;
+            ud = ur;
+            ur = Ur(max(dr - 1, 0));
+        }
+    }
+}
+
+// the product of the last depth is in R (flip = 0) or Rn (flip = 1)
+__global__ void big_out_kernel(BigBatch B, int flip)
+{
+    const BigCore &C = B.c[blockIdx.z];
+    const BigLayout Y = big_layout(B.L, B.L0);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < C.k) C.R[x] = C.ws[(flip ? Y.Rn : Y.R) + x];
 }
 
 // ---------------------------------------------------------------- slicing
@@ -652,11 +1025,15 @@ __global__ void assemble_kernel(AssembleBatch B)
 
 using namespace lcsb;
 
+// Leaves of a big core: 2^14 points (the LDS kernel's largest core).
+static int big_leaf_L(int L) { return L - 1 < 14 ? L - 1 : 14; }
+
 size_t lcsb_core_scratch_words(int32_t k)
 {
     const int L = lcsb_core_L(k);
     const size_t K = (size_t)1 << L;
-    return (size_t)9 * K + 1 + (size_t)2 * L * K;
+    if (K <= (size_t)kLdsCoreK) return (size_t)2 * L * K;   // parents of P and Q per depth
+    return big_layout(L, big_leaf_L(L)).total;
 }
 
 // Padded core size 2^L: at least 64 (the LDS kernel ranks keys by wave
@@ -668,32 +1045,78 @@ int lcsb_core_L(int32_t k)
     return L;
 }
 
+// The big-core sequence for up to kBigBatch products of one padded size.
+static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
+{
+    const int L = B.L, L0 = B.L0, d0 = L - L0, K = 1 << L;
+    const dim3 lin((unsigned)(K / 256), 1, (unsigned)cnt), tiles((unsigned)(K / kSplitTile), 2, (unsigned)cnt);
+    hipLaunchKernelGGL(big_init_kernel, lin, dim3(256), 0, stream, B);
+    for (int d = 0; d < d0; ++d) {
+        hipLaunchKernelGGL(big_split_count_kernel, tiles, dim3(256), 0, stream, B, d);
+        hipLaunchKernelGGL(big_split_scan_kernel, dim3(1u << d, 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
+        hipLaunchKernelGGL(big_split_scatter_kernel, tiles, dim3(256), 0, stream, B, d);
+    }
+    hipLaunchKernelGGL(big_leaf_kernel, dim3(1u << d0, 1, (unsigned)cnt), dim3(kThreads), lds_core_bytes(1 << L0),
+                       stream, B);
+    int flip = 0;
+    for (int d = d0 - 1; d >= 0; --d) {
+        const BigDepth D = big_depth(L, d);
+        const int lines = (K / D.s) * D.nb, pieces = (K / D.s) * (2 * D.nb - 1);
+        hipLaunchKernelGGL(big_lift_kernel, lin, dim3(256), 0, stream, B, d, flip);
+        hipLaunchKernelGGL(big_hist_kernel, dim3((unsigned)(K / D.w), 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
+        hipLaunchKernelGGL(big_bound_kernel, dim3((unsigned)((lines + 255) / 256), 2, (unsigned)cnt), dim3(256), 0,
+                           stream, B, d);
+        hipLaunchKernelGGL(big_sweep_kernel, dim3((unsigned)((lines + 3) / 4), 2, (unsigned)cnt), dim3(256), 0, stream,
+                           B, d);
+        hipLaunchKernelGGL(big_walk_kernel, dim3((unsigned)((pieces + 3) / 4), 1, (unsigned)cnt), dim3(256), 0, stream,
+                           B, d, flip);
+        flip ^= 1;
+    }
+    hipLaunchKernelGGL(big_out_kernel, lin, dim3(256), 0, stream, B, flip);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)braid_core_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)(2 * kMaxCoreK * sizeof(uint16_t))) != hipSuccess ||
-            hipFuncSetAttribute((const void *)braid_core_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void *)braid_core_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_core_bytes(kLdsCoreK)) != hipSuccess ||
+            hipFuncSetAttribute((const void *)big_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_core_bytes(kLdsCoreK)) != hipSuccess)
             return -5;
         attr_set = true;
     }
-    // descriptors travel by value in the kernel arguments
-    for (int base = 0; base < count; base += kCoreBatch) {
+    // small cores: one workgroup each, descriptors by value in the arguments
+    std::vector<BraidCoreDesc> small;
+    std::vector<BraidCoreDesc> big;
+    for (int i = 0; i < count; ++i) {
+        if (descs[i].L > kMaxCoreL) return -34;
+        ((1 << descs[i].L) <= kLdsCoreK ? small : big).push_back(descs[i]);
+    }
+    for (size_t base = 0; base < small.size(); base += kCoreBatch) {
         CoreBatch batch{};
-        const int cnt = count - base < kCoreBatch ? count - base : kCoreBatch;
+        const int cnt = (int)std::min<size_t>(small.size() - base, kCoreBatch);
         int max_K = 1;
         for (int i = 0; i < cnt; ++i) {
-            batch.d[i] = descs[base + i];
+            batch.d[i] = small[base + (size_t)i];
             if ((1 << batch.d[i].L) > max_K) max_K = 1 << batch.d[i].L;
         }
-        if (max_K > kMaxCoreK) return -34;
-        if (max_K <= kLdsCoreK)
-            hipLaunchKernelGGL(braid_core_lds_kernel, dim3(cnt), dim3(kThreads), lds_core_bytes(max_K), stream, batch);
-        else
-            hipLaunchKernelGGL(braid_core_kernel, dim3(cnt), dim3(kThreads), (size_t)2 * max_K * sizeof(uint16_t),
-                               stream, batch);
+        hipLaunchKernelGGL(braid_core_lds_kernel, dim3(cnt), dim3(kThreads), lds_core_bytes(max_K), stream, batch);
+    }
+    // big cores: products of one padded size share each launch of the sequence
+    std::sort(big.begin(), big.end(), [](const BraidCoreDesc &x, const BraidCoreDesc &y) { return x.L < y.L; });
+    for (size_t i = 0; i < big.size();) {
+        BigBatch B{};
+        B.L = big[i].L;
+        B.L0 = big_leaf_L(B.L);
+        int cnt = 0;
+        while (i < big.size() && big[i].L == B.L && cnt < kBigBatch) {
+            B.c[cnt++] = BigCore{big[i].P, big[i].Q, big[i].R, big[i].scratch, big[i].k};
+            ++i;
+        }
+        const int rc = launch_big(B, cnt, stream);
+        if (rc) return rc;
     }
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }

@@ -22,8 +22,8 @@ def test_core_random_permutations(ctx):
             assert np.array_equal(got, exp), (k, glue)
 
 
-# 16384 is the largest core kept wholly in LDS; 20000 and 32768 take the
-# global-memory form of the same kernel (16-bit entries up to K = 32768)
+# 16384 is the largest core kept wholly in one workgroup's LDS; larger
+# cores run over the whole GPU (top depths device-wide, leaves of 2^14)
 @pytest.mark.parametrize("k", [1, 2, 31, 32, 33, 100, 1000, 4097, 16384, 16385, 20000, 32768])
 def test_core_sizes(ctx, k):
     rng = np.random.default_rng(k)
@@ -71,3 +71,62 @@ def test_k_zero_and_bad_input(ctx):
     with pytest.raises(lcs_amd.LcsError) as e:
         ctx.braid_multiply(np.array([0, 0], np.int32), q, 1, False)
     assert e.value.rc == -22
+
+
+@pytest.mark.parametrize("k", [32769, 65536, 100000, 1 << 18])
+def test_big_core_random_permutations(ctx, k):
+    """Cores above 2^14 (the whole-GPU steady ant) against the oracle's
+    staggered_multiply, both glue directions, with p and q wider than k."""
+    rng = np.random.default_rng(k)
+    print(f"big core k={k}", flush=True)
+    for glue in (False, True):
+        ps, qs = k + int(rng.integers(0, 2000)), k + int(rng.integers(0, 2000))
+        p = rng.permutation(ps).astype(np.int32)
+        q = rng.permutation(qs).astype(np.int32)
+        got = ctx.braid_multiply(p, q, k, glue)
+        assert np.array_equal(got, oracle.staggered_multiply(p, q, k, glue)), (k, glue)
+
+
+def test_big_core_structured_permutations(ctx):
+    """Permutations far from random (identity, reversal, shifts, few swaps)
+    make the ant's path hug the box edges: every piece must still join."""
+    k = 1 << 16
+    rng = np.random.default_rng(3)
+    ident = np.arange(k, dtype=np.int32)
+    near = ident.copy()
+    for _ in range(300):
+        i, j = rng.integers(0, k, 2)
+        near[i], near[j] = near[j], near[i]
+    cases = [(ident, ident[::-1].copy()), (ident[::-1].copy(), ident[::-1].copy()), (np.roll(ident, 4321), near),
+             (near, rng.permutation(k).astype(np.int32)), (ident[::-1].copy(), ident)]
+    for p, q in cases:
+        assert np.array_equal(ctx.braid_multiply(p, q, k, False), oracle.staggered_multiply(p, q, k, False))
+
+
+def test_big_composition_m_2p20(ctx):
+    """P(a, b1 b2) = P(a, b1) (x) P(a, b2) glued on k = m = 2^20, the strip
+    permutations from GPU combs and the product from the GPU steady ant,
+    against the oracle's comb of the whole pair."""
+    w = lcs_amd.workload_single("mutated2^20", seed=2)
+    a, b = w.a, w.b[: 3 * 8192]
+    b1, b2 = b[:10000], b[10000:]
+    print("composition m=2^20: combs", flush=True)
+    P1 = ctx.comb_permutation(a, b1, lcs_amd.schedule(strips=1))[0]
+    P2 = ctx.comb_permutation(a, b2, lcs_amd.schedule(strips=1))[0]
+    print("composition m=2^20: product", flush=True)
+    got = ctx.braid_multiply(P1, P2, a.size, False)
+    ho, vo = oracle.comb(a, b, order="simd")
+    assert np.array_equal(got, oracle.from_strands(ho, vo)[0])
+
+
+@pytest.mark.parametrize("m,n,strips", [(1 << 16, 1 << 17, 4), (100000, 150000, 3), (1 << 20, 1 << 15, 2)])
+def test_strips_above_32768_rows(ctx, m, n, strips):
+    """Explicit column strips for pairs with m > 32768 (merged by the
+    whole-GPU steady ant, k = m) give the oracle's strands."""
+    rng = np.random.default_rng(m + n)
+    a = rng.integers(0, 4, m).astype(np.int32)
+    b = rng.integers(0, 4, n).astype(np.int32)
+    print(f"strips m={m} n={n} strips={strips}", flush=True)
+    h, v, _ = ctx.comb(a, b, lcs_amd.schedule(strips=strips))
+    ho, vo = oracle.comb(a, b, order="simd")
+    assert np.array_equal(h, ho) and np.array_equal(v, vo)
