@@ -134,6 +134,22 @@ int lcs_comb_permutation(lcs_ctx *ctx, const int32_t *a, int32_t m, const int32_
 int lcs_comb_device(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
                     const lcs_schedule *sched, int32_t *d_h_out, int32_t *d_v_out, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same comb as lcs_comb_device (one strip, the
+ * band-chain kernels) without a context: its ring, descriptor, task list and
+ * control words live in a static device array of about 32.5 MiB per GPU,
+ * held until the process ends, so the call allocates nothing and copies
+ * nothing from host memory.  Limits: n <= 2^21 columns and at most 65536
+ * band tasks (LCS_ERANGE), strips > 1 is LCS_EINVAL.  Asynchronous on
+ * `stream` (a hipStream_t; NULL = the null stream) of the current device;
+ * d_status (nullable, device int32) receives LCS_OK or LCS_ETIMEDOUT once
+ * the launch's kernels are done.  Every call on one GPU shares that GPU's
+ * array: calls whose kernels overlap on one GPU race, so the caller orders
+ * them (one stream, or events). */
+int lcs_comb_device_noalloc(const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n, const lcs_schedule *sched,
+                            int32_t *d_h_out, int32_t *d_v_out, int32_t *d_status, void *stream);
+
 /* The residual fixup of the reference as a GPU operation: comb the
  * sub-rectangle rows [i0, i0+isize) x columns [j0, j0+jsize) of the grid in
  * place, starting from the strands currently held in h/v (host pointers,
