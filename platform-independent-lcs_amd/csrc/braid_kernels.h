@@ -31,6 +31,9 @@ struct AssembleArgs {
 };
 
 size_t lcsb_core_scratch_words(int32_t k);
+// One-time kernel attribute setup (called by the launches; call it before
+// capturing a launch sequence in a graph).
+int lcsb_init(void);
 int lcsb_core_L(int32_t k);
 // One workgroup per descriptor (host array, copied into the launch arguments).
 int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream);

@@ -30,6 +30,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "braid_kernels.h"
@@ -1025,14 +1026,29 @@ __global__ void assemble_kernel(AssembleBatch B)
 
 using namespace lcsb;
 
-// Leaves of a big core: 2^14 points (the LDS kernel's largest core).
-static int big_leaf_L(int L) { return L - 1 < 14 ? L - 1 : 14; }
+// Cores of 2^L points with L >= kBigMinL run over the whole GPU, with
+// leaves of at most 2^kBigLeafMaxL points (at least 2^10: the device-wide
+// partitions work on tiles of 2048 keys inside nodes of 2^(leaf+1)).
+// Measured on the Zipf pair's five merge levels (k = 2^14, 16 .. 1 products
+// per level): one workgroup per core 15.6 ms for the whole pair; whole-GPU
+// cores with leaves of 2^10 / 2^11 / 2^12 points 12.5 / 12.3 / 12.5 ms.
+#ifndef LCS_BIG_MIN_L
+#define LCS_BIG_MIN_L 12
+#endif
+#ifndef LCS_BIG_LEAF_L
+#define LCS_BIG_LEAF_L 11
+#endif
+constexpr int kBigMinL = LCS_BIG_MIN_L;
+constexpr int kBigLeafMaxL = LCS_BIG_LEAF_L;
+static_assert(kBigMinL >= 11 && kBigLeafMaxL >= 10 && kBigLeafMaxL <= 14, "big-core leaf sizes");
+static int big_leaf_L(int L) { return L - 1 < kBigLeafMaxL ? L - 1 : kBigLeafMaxL; }
+static bool is_big(int L) { return L >= kBigMinL; }
 
 size_t lcsb_core_scratch_words(int32_t k)
 {
     const int L = lcsb_core_L(k);
     const size_t K = (size_t)1 << L;
-    if (K <= (size_t)kLdsCoreK) return (size_t)2 * L * K;   // parents of P and Q per depth
+    if (!is_big(L)) return (size_t)2 * L * K;   // parents of P and Q per depth
     return big_layout(L, big_leaf_L(L)).total;
 }
 
@@ -1076,23 +1092,31 @@ static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
+// The dynamic LDS limit of the core kernels, set once per process (before
+// any capture: it is not a stream operation).
+int lcsb_init(void)
 {
-    static bool attr_set = false;
-    if (!attr_set) {
+    static std::once_flag once;
+    static int rc = 0;
+    std::call_once(once, [] {
         if (hipFuncSetAttribute((const void *)braid_core_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_core_bytes(kLdsCoreK)) != hipSuccess ||
             hipFuncSetAttribute((const void *)big_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_core_bytes(kLdsCoreK)) != hipSuccess)
-            return -5;
-        attr_set = true;
-    }
+            rc = -5;
+    });
+    return rc;
+}
+
+int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
+{
+    if (lcsb_init()) return -5;
     // small cores: one workgroup each, descriptors by value in the arguments
     std::vector<BraidCoreDesc> small;
     std::vector<BraidCoreDesc> big;
     for (int i = 0; i < count; ++i) {
         if (descs[i].L > kMaxCoreL) return -34;
-        ((1 << descs[i].L) <= kLdsCoreK ? small : big).push_back(descs[i]);
+        (is_big(descs[i].L) ? big : small).push_back(descs[i]);
     }
     for (size_t base = 0; base < small.size(); base += kCoreBatch) {
         CoreBatch batch{};
