@@ -4,7 +4,7 @@
 #include <stdint.h>
 
 constexpr int kMlpMaxH = 64;   // hidden width limit (per-thread arrays)
-constexpr int kMlpMaxO = 16;   // output width limit
+constexpr int kMlpMaxO = 32;   // output width limit (one predicted cost per candidate schedule)
 
 struct MlpShape {
     int32_t I, H, O;

@@ -5,7 +5,7 @@ The reference picks its kernel shape by name at compile time
 (tests/test_prepare.ipynb).  Here the library's autotuner
 (lcs_autotune_measure) times every candidate schedule on synthetic pairs of
 a shape; a table of such timings trains the cost model (lcs_mlp_*: an fp32
-MLP 4 -> 64 -> 64 -> len(CANDIDATES), trained and evaluated by the library's
+MLP 4 -> 64 -> 64 -> one cost per candidate, trained and evaluated by the library's
 own HIP kernels) to predict log2 cost per candidate from
 
     (log2 m, log2 n, log2 alphabet, mutation rate)        (scaled, see features)
@@ -35,6 +35,17 @@ TILES = (1, 2, 4, 8, 16, 32)
 # every (tile height, variant) pair, one wave per SIMD, no column strips
 CANDIDATES = [schedule(tile_m=t, waves_per_simd=1, variant=v, strips=1) for v in (VARIANT_CHAIN, VARIANT_CHAIN_LDS)
               for t in TILES]
+# a batch also prices the 16-bit packed chain (two pairs per wave); for a
+# single pair it is the register chain, so single-pair tables leave it out
+VARIANT_PACKED16 = 3
+BATCH_CANDIDATES = CANDIDATES + [schedule(tile_m=t, waves_per_simd=1, variant=VARIANT_PACKED16, strips=1)
+                                 for t in TILES]
+MODE_CANDIDATES = {0: CANDIDATES, 1: BATCH_CANDIDATES}
+
+
+def table_candidates(table: dict):
+    """The candidate schedules a table's cost columns belong to."""
+    return [Schedule(*c) for c in table["candidates"]]
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 TABLES = {1: os.path.join(DATA_DIR, "autotune_batch_mi355x.json"),
           0: os.path.join(DATA_DIR, "autotune_single_mi355x.json")}
@@ -78,13 +89,14 @@ def record(ctx: Context, shapes, mode: int = 1, reps: int = 1, seed: int = 1, lo
     t_start = time.perf_counter()
     for i, sh in enumerate(shapes):
         t0 = time.perf_counter()
-        cost = ctx.autotune(sh["m"], sh["n"], sh["alphabet"], sh["rate"], CANDIDATES, mode=mode, reps=reps,
+        cost = ctx.autotune(sh["m"], sh["n"], sh["alphabet"], sh["rate"], MODE_CANDIDATES[mode], mode=mode, reps=reps,
                             seed=seed + i)
         rows.append(dict(sh, cost_ms=[float(c) for c in cost], tune_s=round(time.perf_counter() - t0, 3)))
         if log:
             log(f"shape {i + 1}/{len(shapes)} m={sh['m']} n={sh['n']} a={sh['alphabet']} "
                 f"best={int(np.argmin(cost))} {time.perf_counter() - t0:.2f}s")
-    return {"mode": mode, "candidates": [[c.tile_m, c.waves_per_simd, c.variant, c.strips] for c in CANDIDATES],
+    return {"mode": mode,
+            "candidates": [[c.tile_m, c.waves_per_simd, c.variant, c.strips] for c in MODE_CANDIDATES[mode]],
             "rows": rows, "device": ctx.device_name(), "seconds": round(time.perf_counter() - t_start, 1)}
 
 
@@ -173,10 +185,12 @@ def evaluate(ctx: Context, table: dict, folds: int = 4, steps: int = 4000, seed:
             "exhaustive_tune_s_per_shape": tune_s}
 
 
-def choose_schedules(mlp: Mlp, m, n, alphabet, rate):
-    """Model's schedule per pair (a list of Schedule)."""
+def choose_schedules(mlp: Mlp, m, n, alphabet, rate, candidates=None):
+    """Model's schedule per pair (a list of Schedule); `candidates` are the
+    schedules of the model's outputs (its table's, CANDIDATES by default)."""
+    cands = candidates if candidates is not None else CANDIDATES
     idx = mlp.choose(features(m, n, alphabet, rate))
-    return [CANDIDATES[int(i)] for i in idx], idx
+    return [cands[int(i)] for i in idx], idx
 
 
 def _main():
@@ -198,6 +212,8 @@ def _main():
         shapes = sample_shapes(args.shapes, args.seed, args.lo, args.hi,
                                tuple(int(x) for x in args.alphabets.split(",")))
         table = record(ctx, shapes, mode=args.mode, log=lambda s: print(s, flush=True))
+        table["recorded_with"] = (f"python -m lcs_amd.tuning record --mode {args.mode} --shapes {args.shapes} "
+                                  f"--seed {args.seed} --lo {args.lo} --hi {args.hi} --alphabets {args.alphabets}")
         os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
         with open(args.out, "w") as f:
             json.dump(table, f, indent=1)

@@ -56,7 +56,8 @@ def test_learned_schedules_do_not_change_results(ctx):
     mlp = tuning.train(ctx, table, steps=1000)
     pairs = lcs_amd.workload_batch(64, seed=31, lo=256, hi=4096)
     scheds, _ = tuning.choose_schedules(mlp, [p.a.size for p in pairs], [p.b.size for p in pairs],
-                                        [p.alphabet for p in pairs], [p.rate for p in pairs])
+                                        [p.alphabet for p in pairs], [p.rate for p in pairs],
+                                        tuning.table_candidates(table))
     h1, v1 = ctx.batch_comb([p.a for p in pairs], [p.b for p in pairs], scheds=scheds)
     h2, v2 = ctx.batch_comb([p.a for p in pairs], [p.b for p in pairs])
     assert all(np.array_equal(x, y) for x, y in zip(h1, h2))
@@ -104,9 +105,10 @@ def test_selection_order_table_model_autotune_default():
         if table is None:
             pytest.skip("no autotune table to train the model on")
         mlp = tuning.train(c, table, steps=500)
-        c.set_cost_model(lcs_amd.TUNE_SINGLE, mlp, tuning.CANDIDATES)
+        cands = tuning.table_candidates(table)
+        c.set_cost_model(lcs_amd.TUNE_SINGLE, mlp, cands)
         s, src = c.choose_schedule(lcs_amd.TUNE_SINGLE, 6000, 5000, alphabet=4, rate=0.1)
-        expect = tuning.CANDIDATES[int(mlp.choose(tuning.features([6000], [5000], [4], [0.1]))[0])]
+        expect = cands[int(mlp.choose(tuning.features([6000], [5000], [4], [0.1]))[0])]
         assert src == "model" and _same_schedule(s, expect)
         s, src = c.choose_schedule(lcs_amd.TUNE_SINGLE, 3000, 5000)
         assert src == "table" and _same_schedule(s, row)

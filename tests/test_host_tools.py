@@ -14,6 +14,10 @@ def test_candidates_cover_every_tile_and_variant():
     seen = {(c.tile_m, c.variant) for c in tuning.CANDIDATES}
     assert seen == {(t, v) for t in (1, 2, 4, 8, 16, 32) for v in (1, 2)}
     assert all(c.waves_per_simd == 1 and c.strips == 1 for c in tuning.CANDIDATES)
+    # a batch also prices the packed chain; the first candidates are the same
+    seen = {(c.tile_m, c.variant) for c in tuning.BATCH_CANDIDATES}
+    assert seen == {(t, v) for t in (1, 2, 4, 8, 16, 32) for v in (1, 2, 3)}
+    assert tuning.BATCH_CANDIDATES[:12] == tuning.CANDIDATES
 
 
 def test_features_are_scaled_inputs():
@@ -48,9 +52,10 @@ def test_committed_timing_tables_are_well_formed():
     for mode in (1, 0):
         t = tuning.load_table(mode)
         assert t["mode"] == mode
-        assert t["candidates"] == [[c.tile_m, c.waves_per_simd, c.variant, c.strips] for c in tuning.CANDIDATES]
+        cands = tuning.MODE_CANDIDATES[mode]
+        assert t["candidates"] == [[c.tile_m, c.waves_per_simd, c.variant, c.strips] for c in cands]
         X, T, M, C = tuning.table_arrays(t)
-        assert X.shape == (len(t["rows"]), 4) and C.shape == (len(t["rows"]), len(tuning.CANDIDATES))
+        assert X.shape == (len(t["rows"]), 4) and C.shape == (len(t["rows"]), len(cands))
         assert M.all() and np.isfinite(C).all() and (C > 0).all()
 
 
