@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--check", action="store_true", help="cross-check two schedules bit-for-bit")
     ap.add_argument("--no-zipf", action="store_true", help="skip the m=2^14, n=2^22 Zipf leg")
     ap.add_argument("--no-semilocal", action="store_true", help="skip the semi-local query leg")
+    ap.add_argument("--no-merge", action="store_true", help="skip the strips + steady-ant merge leg")
     ap.add_argument("--no-llcs", action="store_true", help="skip the bit-parallel LCS-length leg")
     ap.add_argument("--batch-pairs", type=int, default=10000, help="pairs of the batched leg (0: skip it)")
     ap.add_argument("--serve-requests", type=int, default=600, help="requests of the served leg (0: skip it)")
@@ -70,12 +71,18 @@ def host_cpus():
 
 
 def _strip_rate(fn, m: int, b, target_s: float, min_cols: int):
-    """Times fn(b[:cols]) on a full-height strip sized for ~target_s."""
+    """Times fn(b[:cols]) on a full-height strip sized for ~target_s: the
+    strip grows 4x until a run takes a quarter of the target (thread start-up
+    and caches no longer dominate), then is sized from that run's rate."""
     cols = min(len(b), min_cols)
-    t0 = time.perf_counter()
-    fn(b[:cols])
-    rate = m * cols / max(time.perf_counter() - t0, 1e-6)
-    cols = int(min(len(b), max(min_cols, rate * target_s / m)))
+    while True:
+        t0 = time.perf_counter()
+        fn(b[:cols])
+        dt = max(time.perf_counter() - t0, 1e-6)
+        if dt >= target_s / 4 or cols >= len(b):
+            break
+        cols = min(len(b), cols * 4)
+    cols = int(min(len(b), max(cols, cols * target_s / dt)))
     t0 = time.perf_counter()
     fn(b[:cols])
     return cols, time.perf_counter() - t0
@@ -223,19 +230,30 @@ def bench_batch(ctx, npairs: int):
            "batched_heuristic_kernel_ms": round(heur_kms, 3), "batched_heuristic_bit_exact": heur_exact}
     mlp, info = learned_model(ctx, 1)
     if mlp is not None:
+        cands = tuning.table_candidates(tuning.load_table(1))
         scheds, idx = tuning.choose_schedules(mlp, lay.m, lay.n, [p.alphabet for p in pairs],
-                                              [p.rate for p in pairs])
+                                              [p.rate for p in pairs], cands)
         l_s, l_kms, l_exact = batched(scheds)
         out.update({"batched_learned_pairs_per_s": round(npairs / l_s, 1), "batched_learned_s": round(l_s, 4),
                     "batched_learned_kernel_ms": round(l_kms, 3), "batched_learned_bit_exact": l_exact,
-                    "learned_choice_histogram": {f"tm{tuning.CANDIDATES[i].tile_m}v{tuning.CANDIDATES[i].variant}":
+                    "learned_choice_histogram": {f"tm{cands[i].tile_m}v{cands[i].variant}":
                                                  int(c) for i, c in zip(*np.unique(idx, return_counts=True))},
                     "cost_model": info})
         mlp.close()
         best_s = min(heur_s, l_s)
+        tiles = [sc.tile_m for sc in scheds]
     else:
         out["cost_model"] = "no autotune table: heuristic schedules only"
         best_s = heur_s
+        tiles = [0] * npairs
+    # 16-bit strands, two pairs per wave (every pair here has m + n <= 32768)
+    packed = [lcs_amd.schedule(tile_m=t, variant=lcs_amd.VARIANT_PACKED16) for t in tiles]
+    p_s, p_kms, p_exact = batched(packed)
+    out.update({"batched_packed16_pairs_per_s": round(npairs / p_s, 1), "batched_packed16_s": round(p_s, 4),
+                "batched_packed16_kernel_ms": round(p_kms, 3), "batched_packed16_bit_exact": p_exact,
+                "batched_packed16_tiles": "learned" if mlp is not None else "heuristic"})
+    if p_exact:
+        best_s = min(best_s, p_s)
     out["speedup_vs_plain"] = round(plain_s / best_s, 2)
     out["batched_gcups"] = round(cells / best_s / 1e9, 1)
     return out
@@ -279,6 +297,39 @@ def bench_zipf(ctx, steps: int = 3):
             "gcups": round(cells / dt / 1e9, 1), "ms": round(dt * 1e3, 2),
             "one_strip_gcups": round(cells / dt1 / 1e9, 1), "strips_vs_one_strip_bit_exact": exact,
             "llcs": int((v < m).sum())}
+
+
+def bench_tiled_merge(ctx, da, db, m: int, n: int, h_ref, v_ref, strips=(1, 2, 4, 8)):
+    """The m = n = 2^20 pair with S column strips combed in one chain launch
+    and the strip permutations merged pairwise by the whole-GPU steady ant
+    (k = m = 2^20, log2 S levels); S = 1 is the band chain alone.  Each
+    result is compared bit for bit with the headline's."""
+    L = lcs_amd.lib()
+    dh, dv = H.DeviceArray(m), H.DeviceArray(n)
+    out = {"workload": "single pair m=n=2^20, S column strips + steady-ant merges (k = m)"}
+    for S in strips:
+        sched = lcs_amd.schedule(strips=S)
+
+        def run():
+            rc = L.lcs_comb_device(ctx.handle, da.ptr, m, db.ptr, n, sched, dh.ptr, dv.ptr, None)
+            if rc:
+                raise lcs_amd.LcsError("lcs_comb_device", rc)
+            H.synchronize()
+            st = ctx.status()
+            if st:
+                raise lcs_amd.LcsError("lcs_ctx_status", st)
+            return ctx.info()[1]
+
+        run()
+        ks = [run() for _ in range(2)]
+        exact = bool(np.array_equal(dh.numpy(), h_ref) and np.array_equal(dv.numpy(), v_ref))
+        k = min(ks)
+        out[f"strips{S}"] = {"kernel_ms": round(k, 3), "gcups": round(float(m) * n / (k / 1e3) / 1e9, 1),
+                             "chain_ms": round(ctx.chain_ms(), 3), "merge_levels": int(np.log2(S)),
+                             "bit_exact_vs_headline": exact}
+    dh.free()
+    dv.free()
+    return out
 
 
 def bench_semilocal(ctx, r2c, m: int, n: int, llcs: int, nq: int = 1 << 22):
@@ -348,43 +399,78 @@ def bench_serve(ctx, nreq: int, rate: float):
     [1024, 262144], alphabet 4 or 20 (50/50), B from A by the same mutation
     model; admitted by an HBM budget into overlapping micro-batches, the
     schedule of each request chosen by the learned cost model.  Latency is
-    submit-to-result on the host."""
+    submit-to-result on the host.  Beside the offered rate: the same
+    requests all submitted at once (the server's capacity), higher offered
+    rates (p99 at each), and the plain path under the same arrivals (one
+    synchronous lcs_comb per request, in arrival order)."""
     from lcs_amd import tuning
     pairs = lcs_amd.workload_batch(nreq, seed=9, lo=1024, hi=262144, alphabet=(4, 20))
     srv = lcs_amd.Server(ctx, hbm_budget=8 << 30, max_inflight=4)
     mlp, info = learned_model(ctx, 0)
     if mlp is not None:
-        srv.set_cost_model(mlp, tuning.CANDIDATES)
+        srv.set_cost_model(mlp, tuning.table_candidates(tuning.load_table(0)))
     feats = tuning.features([p.a.size for p in pairs], [p.b.size for p in pairs], [p.alphabet for p in pairs],
                             [p.rate for p in pairs])
-    rng = np.random.default_rng(5)
-    gaps = rng.exponential(1.0 / rate, nreq)
-    ids, t_sub = [], []
-    t0 = time.perf_counter()
-    next_t = t0
-    for p, f, gap in zip(pairs, feats, gaps):
-        next_t += gap
-        while time.perf_counter() < next_t:
-            time.sleep(max(0.0, min(next_t - time.perf_counter(), 0.002)))
-        ids.append(srv.submit(p.a, p.b, None, f))
-    outs = [srv.wait(i, timeout_ms=120000) for i in ids]
-    wall = time.perf_counter() - t0
-    lat = np.array([srv.times(i)[1] for i in ids]) / 1e3   # ms
-    st = srv.stats()
+    gaps = np.random.default_rng(5).exponential(1.0, nreq)   # unit-rate Poisson gaps, scaled per run
+    cells = float(sum(p.a.size * p.b.size for p in pairs))
+
+    def served(r):
+        """r requests/s (None: every request at t = 0); (wall s, latencies ms, stats, outputs)."""
+        ids = []
+        t0 = time.perf_counter()
+        next_t = t0
+        for p, f, gap in zip(pairs, feats, gaps):
+            if r:
+                next_t += gap / r
+                while time.perf_counter() < next_t:
+                    time.sleep(max(0.0, min(next_t - time.perf_counter(), 0.002)))
+            ids.append(srv.submit(p.a, p.b, None, f))
+        outs = [srv.wait(i, timeout_ms=120000) for i in ids]
+        wall = time.perf_counter() - t0
+        lat = np.array([srv.times(i)[1] for i in ids]) / 1e3
+        st = srv.stats()
+        for i in ids:
+            srv.release(i)
+        return wall, lat, st, outs
+
+    def pct(lat):
+        return round(float(np.percentile(lat, 50)), 2), round(float(np.percentile(lat, 99)), 2)
+
+    wall, lat, st, outs = served(rate)
     # a sample of results against the same pair alone (bit-exact)
     exact = True
     for p, o in list(zip(pairs, outs))[:: max(1, nreq // 12)]:
         h1, v1, _ = ctx.comb(p.a, p.b)
         exact = exact and np.array_equal(o[0], h1) and np.array_equal(o[1], v1)
-    for i in ids:
-        srv.release(i)
+    p50, p99 = pct(lat)
+    sat_wall, sat_lat, _, _ = served(None)
+    sweep = {}
+    for r in (2 * rate, 4 * rate):
+        w2, l2, _, _ = served(r)
+        sweep[f"{int(r)}/s"] = {"pairs_per_s": round(nreq / w2, 1), "p50_ms": pct(l2)[0], "p99_ms": pct(l2)[1]}
     srv.close()
     if mlp is not None:
         mlp.close()
-    cells = float(sum(p.a.size * p.b.size for p in pairs))
+    # the plain path under the same arrivals: one synchronous call per request
+    plain_lat = []
+    t0 = time.perf_counter()
+    arrival = t0
+    for p, gap in zip(pairs, gaps):
+        arrival += gap / rate
+        while time.perf_counter() < arrival:
+            time.sleep(max(0.0, min(arrival - time.perf_counter(), 0.002)))
+        ctx.comb(p.a, p.b)
+        plain_lat.append((time.perf_counter() - arrival) * 1e3)
+    plain_wall = time.perf_counter() - t0
+    pp50, pp99 = pct(np.array(plain_lat))
     return {"workload": f"{nreq} requests, Poisson {rate}/s, lengths log-uniform [1024,262144], alphabet 4|20",
             "pairs_per_s": round(nreq / wall, 1), "gcups": round(cells / wall / 1e9, 1),
-            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "p50_ms": p50, "p99_ms": p99,
+            "plain_pairs_per_s": round(nreq / plain_wall, 1), "plain_p50_ms": pp50, "plain_p99_ms": pp99,
+            "capacity_pairs_per_s": round(nreq / sat_wall, 1),
+            "capacity_gcups": round(cells / sat_wall / 1e9, 1),
+            "capacity_p99_ms": pct(sat_lat)[1],
+            "higher_rates": sweep,
             "micro_batches": st["batches"], "max_inflight": st["max_inflight"],
             "bit_exact_vs_alone_sample": bool(exact), "cost_model": info or "none (heuristic schedules)"}
 
@@ -419,7 +505,7 @@ def main():
         st = ctx.status()
         if st:
             raise lcs_amd.LcsError("lcs_ctx_status", st)
-        return ctx.info()[1]
+        return ctx.info()[1], ctx.chain_ms()
 
     for _ in range(args.warmup):
         step()
@@ -443,9 +529,12 @@ def main():
     cells = float(m) * float(n)
     ms_step = wall / args.steps * 1e3
     gcups = cells * args.steps / wall / 1e9
-    k_ms = float(np.mean(kernel_ms))
+    # the dominant kernel is the band chain (one launch combs every strip);
+    # the rest of a step is the strip permutations and steady-ant merges
+    k_ms = float(np.mean([t for t, _ in kernel_ms]))
+    chain_ms = float(np.mean([c for _, c in kernel_ms]))
     peak_ops = cus * 64 * PEAK_CLOCK_HZ
-    achieved_ops = OPS_PER_CELL * cells / (k_ms / 1e3)
+    achieved_ops = OPS_PER_CELL * cells / (chain_ms / 1e3)
     pmc_file = os.path.join(ROOT, "profiles", "comb_chain_mutated2^20_pmc.txt")
     traffic = read_traffic(pmc_file)
     clock_ghz = read_profile_value(pmc_file, "GRBM_GUI_ACTIVE / 8 XCDs / kernel time =")
@@ -470,12 +559,16 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (seeded mt19937_64: uniform alphabet-4 A, B = A + 10% subs + 2% indels)",
-        "config": {"workload": f"single pair m=n=2^{args.log2n}, tiled band-chain comb", "m": m, "n": n,
-                   "alphabet": 4, "tile_m": args.tile_m or None, "variant": args.variant or None, "llcs": llcs},
+        "config": {"workload": f"single pair m=n=2^{args.log2n}, tiled band-chain comb + steady-ant merge "
+                               "(library default schedule: two column strips, one merge with k = m)",
+                   "m": m, "n": n, "alphabet": 4, "tile_m": args.tile_m or None, "variant": args.variant or None,
+                   "llcs": llcs},
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 3), "peak": round(peak_ops / 1e12, 3),
                      "unit": "TOP/s (int32 lane-ops, 4 per cell)", "frac": round(achieved_ops / peak_ops, 4),
-                     "traffic": traffic, "kernel_ms": round(k_ms, 3),
-                     "kernel_gcups": round(cells / (k_ms / 1e3) / 1e9, 2),
+                     "traffic": traffic, "kernel": "comb_chain_kernel<16,32> (seed + chain + extract launch)",
+                     "kernel_ms": round(chain_ms, 3), "kernel_gcups": round(cells / (chain_ms / 1e3) / 1e9, 2),
+                     "step_kernels_ms": round(k_ms, 3),
+                     "merge_share": round(max(0.0, 1.0 - chain_ms / k_ms), 4),
                      "measured_clock_ghz": clock_ghz,
                      "frac_at_measured_clock": (round(achieved_ops / (cus * 64 * clock_ghz * 1e9), 4)
                                                 if clock_ghz else None)},
@@ -484,6 +577,8 @@ def main():
     }
     if checked is not None:
         out["cross_schedule_bit_exact"] = checked
+    if not args.no_merge:
+        out["tiled_merge"] = bench_tiled_merge(ctx, da, db, m, n, h, v)
     if not args.no_semilocal:
         out["semilocal"] = bench_semilocal(ctx, r2c, m, n, llcs)
     if not args.no_llcs:

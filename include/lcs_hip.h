@@ -61,6 +61,13 @@ typedef struct lcs_schedule {
 #define LCS_VARIANT_AUTO 0
 #define LCS_VARIANT_CHAIN 1      /* wave64 band chain, registers only         */
 #define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */
+/* Batches only: pairs of a batch with this variant are combed two per wave
+ * with 16-bit strands (pair X in the low, pair Y in the high half of every
+ * register, 5 VALU per 2 cells against 4 per cell).  A pair qualifies when
+ * m, n >= 1, m + n <= 65535 and every symbol lies in [0, 65533]; the others
+ * (and an odd one out) take the register chain.  Same bits as every other
+ * variant.  Elsewhere it means LCS_VARIANT_CHAIN. */
+#define LCS_VARIANT_PACKED16 3
 
 /* ---- context ------------------------------------------------------------
  * Replaces the registry-owned sycl::queue (src/algorithm_registry.cpp:6-41,
@@ -75,6 +82,11 @@ int lcs_ctx_device_name(lcs_ctx *ctx, char *buf, size_t len);
 /* Number of compute units, and the kernel-measured time in milliseconds of
  * the last lcs_comb / lcs_comb_device call (HIP events around the kernels). */
 int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms);
+/* Time in milliseconds of the band-chain launch alone (its seed, chain and
+ * extract kernels, HIP events) within the last single-pair call; for a pair
+ * cut into column strips this leaves out the strip permutations and the
+ * steady-ant merges.  Waits for that launch; LCS_ENOENT before any call. */
+int lcs_ctx_chain_ms(lcs_ctx *ctx, double *chain_ms);
 /* Waits for every kernel the context has enqueued (its own stream and the
  * streams given to the asynchronous entry points) and reports whether any
  * launch since the previous lcs_ctx_status gave up a device-side wait:

@@ -18,6 +18,22 @@ struct PairDesc {
     int32_t m, n, nbands, tile_m;
 };
 
+// Two pairs combed together by the 16-bit packed chain
+// (LCS_VARIANT_PACKED16): pair q (0 = X, low halves; 1 = Y, high halves)
+// reads a[q], b[q] (m_[q], n_[q] int32 symbols in [0, 0xfffd]) and writes
+// h_out[q], v_out[q]; each has m_[q] + n_[q] <= 65535 and m_[q], n_[q] >= 1.
+// m, n are the larger extents; the shorter pair is padded with rows and
+// columns that never match and never swap.
+struct CoupleDesc {
+    const int32_t *a[2];
+    const int32_t *b[2];
+    int32_t *h_out[2];
+    int32_t *v_out[2];
+    int32_t m_[2], n_[2];
+    unsigned long long *E;       // 2*n ring words (tag << 32 | Y strand << 16 | X strand) + 64 trash words
+    int32_t m, n, nbands, tile_m;
+};
+
 // Words of ring a pair of n columns needs.
 inline size_t lcsk_ring_words(int32_t n) { return 2 * (size_t)n + 64; }
 
@@ -26,7 +42,8 @@ inline size_t lcsk_ring_words(int32_t n) { return 2 * (size_t)n + 64; }
 // group_w*tasks[t].y .. +group_w-1 in one workgroup (group_w > 1).  Tasks must list every band after the
 // band above it (same pair), which makes every wait one on a resident wave.
 struct ChainArgs {
-    const PairDesc *pairs;
+    const PairDesc *pairs;       // the register chain and LDS groups
+    const CoupleDesc *couples;   // the packed chain (tasks[t].x indexes couples)
     const int2 *tasks;
     int32_t ntasks;
     int32_t group_w;             // 1: one wave per task (register chain); W > 1: LDS groups of W bands
@@ -35,7 +52,8 @@ struct ChainArgs {
     unsigned long long timeout_ticks;  // 100 MHz realtime ticks per wait
     unsigned long long *trace;   // nullable: [2t] start, [2t+1] end of task t (realtime ticks)
     int32_t trace_cap;           // tasks the trace buffer holds
-    int32_t pad1;
+    int32_t poll_sleep;          // 1: a wave waiting for its inputs sleeps between polls (launches that
+                                 // share SIMDs with other launches' waves)
     unsigned *sticky;            // nullable: the prep kernel ORs the abort flag into it before clearing it
 };
 
@@ -44,6 +62,18 @@ struct ChainArgs {
 int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs, int max_n, int tile_m,
                       int grid_blocks, int block_threads, hipStream_t stream);
 int lcsk_launch_identity(int32_t *h_out, int32_t m, int32_t *v_out, int32_t n, hipStream_t stream);
+// A string whose symbols the packed chain needs in [0, 0xfffd]: bad[item]
+// becomes nonzero when one is outside.
+struct SymbolRange {
+    const int32_t *x;
+    int32_t len, item;
+};
+int lcsk_launch_check16(const SymbolRange *d_ranges, int32_t nranges, int32_t *d_bad, int32_t nitems,
+                        hipStream_t stream);
+// The same sequence (seed, chain, extract) for couples of pairs combed by
+// the 16-bit packed chain.
+int lcsk_launch_chain16(const ChainArgs &args, const CoupleDesc *d_couples, int ncouples, int max_n, int tile_m,
+                        int grid_blocks, int block_threads, hipStream_t stream);
 // row_to_col of the permutation of each pair from its strands (m+n entries
 // each, written at perm + perm_off[p]).
 int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, int32_t *perm,
@@ -51,6 +81,8 @@ int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, i
 // row_to_col (size m+n) of one pair's permutation from its device strands.
 int lcsk_launch_r2c_from_strands(const int32_t *h, int32_t m, const int32_t *v, int32_t n, int32_t *r2c,
                                  hipStream_t stream);
+// dst[0..count) = src[0..count) (a kernel, so it can sit in a captured graph).
+int lcsk_launch_copy(int32_t *dst, const int32_t *src, long long count, hipStream_t stream);
 // Strands from a permutation (row_to_col, size m+n): h_out[l] = row of col
 // n+l, v_out[j] = row of col j.  scratch holds m+n words.
 int lcsk_launch_strands_from_perm(const int32_t *r2c, int32_t m, int32_t n, int32_t *scratch, int32_t *h_out,

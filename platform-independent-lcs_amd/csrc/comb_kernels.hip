@@ -134,10 +134,32 @@ __device__ __forceinline__ void column_cells(const int32_t (&a)[TM], uint32_t (&
     }
 }
 
+// Two cells at once, pair X in the low and pair Y in the high 16 bits of
+// every register (the packed chain): per half, swap iff sa == sb || h > v,
+// as the 32-bit cell.  With e = sa ^ sb, sat(1 - e) is 1 on a match, and
+// sat(h + 1 * 0xffff) = 0xffff replaces h there, so
+//     hx = mad_sat(sat_sub(1, e), 0xffff, h),  hn = min(hx, v),  v' = h ^ v ^ hn
+// is v_xor + v_pk_sub_u16 clamp + v_pk_mad_u16 clamp + v_pk_min_u16 +
+// v_bitop3: 5 VALU for 2 cells (4 per cell in 32 bits).  Strand ids stay
+// below 0xffff, the value a padded column carries.
+typedef unsigned short lcsk_u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void cell16(uint32_t sa, uint32_t sb, uint32_t &h, uint32_t &v)
+{
+    const lcsk_u16x2 one = {1, 1};
+    const lcsk_u16x2 mk = __builtin_elementwise_sub_sat(one, __builtin_bit_cast(lcsk_u16x2, sa ^ sb));
+    uint32_t hx;
+    asm("v_pk_mad_u16 %0, %1, -1, %2 clamp" : "=v"(hx) : "v"(__builtin_bit_cast(uint32_t, mk)), "v"(h));
+    const uint32_t hn = __builtin_bit_cast(
+        uint32_t, __builtin_elementwise_min(__builtin_bit_cast(lcsk_u16x2, hx), __builtin_bit_cast(lcsk_u16x2, v)));
+    v = __builtin_amdgcn_bitop3_b32(h, v, hn, 0x96);
+    h = hn;
+}
+
 // Steps t0 .. t0+C-1 of a band.  COLMASK: some lane's column leaves [0, n).
-template <int TM, int C, bool ROWMASK, bool COLMASK>
+// The conveyors I and IB come back shifted by C lanes.
+template <int TM, int C, bool ROWMASK, bool COLMASK, bool PK = false>
 __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
-                                           uint32_t &v, int32_t &bq, uint32_t &O, uint32_t I, int32_t IB,
+                                           uint32_t &v, int32_t &bq, uint32_t &O, uint32_t &I, int32_t &IB,
                                            int t0, int lane, int n)
 {
 #pragma unroll
@@ -155,7 +177,14 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
             const int j = t0 + s - lane;
             active = (j >= 0) && (j < n);
         }
-        if (active) column_cells<TM, ROWMASK>(a, h, valid, bq, v);
+        if (active) {
+            if constexpr (PK) {
+#pragma unroll
+                for (int k = 0; k < TM; ++k) cell16((uint32_t)a[k], (uint32_t)bq, h[k], v);
+            } else {
+                column_cells<TM, ROWMASK>(a, h, valid, bq, v);
+            }
+        }
     }
 }
 
@@ -164,8 +193,16 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
 // (tag g+1).  The prep kernel seeds slot 0 with the grid's top edge (tag 0)
 // and the extract kernel reads v_out from the last band's slot, so no
 // branch on the band's position sits between a load and its wait.
-template <int TM, int C, bool ROWMASK>
-__device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane, int t_id)
+// PK: the packed chain over a CoupleDesc -- every symbol and strand
+// register holds pair X in its low and pair Y in its high 16 bits.  Rows
+// past a pair's m hold symbol kPadRow16 and strand 0 and columns past its n
+// carry symbol kPadCol16 and strand 0xffff: they never match and never
+// swap, so they pass every strand through unchanged and ROWMASK is not
+// needed.
+constexpr uint32_t kPadRow16 = 0xffffu, kPadCol16 = 0xfffeu;
+
+template <int TM, int C, bool ROWMASK, bool PK, typename Desc>
+__device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, int t_id)
 {
     const int m = P.m, n = P.n;
     const int r0 = g * (kWave * TM) + lane * TM;
@@ -176,16 +213,45 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
 #pragma unroll
     for (int k = 0; k < TM; ++k) {
         const int r = r0 + k;
-        valid[k] = ROWMASK ? (r < m) : true;
-        a[k] = valid[k] ? P.a[r] : 0;
-        h[k] = valid[k] ? (P.h_in ? (uint32_t)P.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
+        if constexpr (PK) {
+            uint32_t sa = 0, hh = 0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const bool in = r < P.m_[q];
+                const uint32_t sym = in ? ((uint32_t)P.a[q][r] & 0xffffu) : kPadRow16;
+                const uint32_t id = in ? (uint32_t)(P.m_[q] - 1 - r) : 0u;
+                sa |= sym << (16 * q);
+                hh |= id << (16 * q);
+            }
+            valid[k] = true;
+            a[k] = (int32_t)sa;
+            h[k] = hh;
+        } else {
+            valid[k] = ROWMASK ? (r < m) : true;
+            a[k] = valid[k] ? P.a[r] : 0;
+            h[k] = valid[k] ? (P.h_in ? (uint32_t)P.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
+        }
     }
     const uint32_t tag_in = (uint32_t)g;
     const unsigned long long tag_out = (unsigned long long)(g + 1) << 32;
     gcword *Ein = (gcword *)(P.E + (size_t)(g & 1) * (size_t)n);
     gword *Eout = (gword *)(P.E + (size_t)((g + 1) & 1) * (size_t)n);
-    gsym *bsym = (gsym *)P.b;
     const int nchunks = (n + kWave + C - 1) / C;
+    // the column symbol of column c (c in [0, n)): for the packed chain both
+    // pairs' symbols, each from a clamped in-bounds address
+    auto load_b = [&](auto c) -> int32_t {
+        if constexpr (PK) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t sym = (uint32_t)((gsym *)P.b[q])[min((int)c, P.n_[q] - 1)] & 0xffffu;
+                x |= ((int)c < P.n_[q] ? sym : kPadCol16) << (16 * q);
+            }
+            return (int32_t)x;
+        } else {
+            return ((gsym *)P.b)[c];
+        }
+    };
 
     // prefetch chunk 0.  Every lane loads from a clamped in-bounds address
     // (no exec masking, no select), so the waits before each chunk cover
@@ -194,7 +260,7 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     // Only lanes < C use their words (the conveyors start at lane 0 and
     // shift C times); lanes >= C load the same addresses, so each chunk
     // fetches C words, not 64.
-    int32_t pB = bsym[min(lane & (C - 1), nm1)];
+    int32_t pB = load_b(min(lane & (C - 1), nm1));
     unsigned long long pE = ld_relaxed(Ein + min(lane & (C - 1), nm1));
     // Publishes the C bottom-edge strands caught in O during the chunk that
     // started at step t: O lanes [64-C, 64) hold columns t-64 .. t-64+C-1.
@@ -203,6 +269,10 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     // lane 64-C + L%C (fetched by one bpermute), so the C/64 copies of each
     // word carry the same value to the same address.  Columns outside [0, n)
     // (the first and last chunks of a band) go to trash words past the ring.
+    // A chunk's words are published during the next chunk: publishing them
+    // right after their chunk (the upper half of O copied down by
+    // v_permlane32_swap instead of the bpermute) was measured slower, 146.1
+    // against 144.4 ms at m=n=2^20.
     static_assert(C > 0 && (C & (C - 1)) == 0 && C <= kWave, "C must be a power of two <= 64");
     gword *trash = (gword *)(P.E + 2 * (size_t)n);
     // INSIDE: every column is known to be in [0, n), so the store needs no
@@ -232,6 +302,9 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
             // back to back without them was measured slower.
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
             do {
+                // a wave that shares its SIMD with other launches' waves
+                // gives up its issue slots between polls
+                if (A.poll_sleep) __builtin_amdgcn_s_sleep(2);
                 if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) return false;
                 if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
                     raise_abort(A.abort_flag);
@@ -254,21 +327,23 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     // register copies between masked and unmasked paths; the interior loop
     // takes two chunks per iteration, which lets the compiler close it
     // without copying the h registers (with one chunk it copied them).
+    // Issuing the prefetch halfway through the chunk instead was measured no
+    // faster.
     using Masked = std::integral_constant<bool, true>;
     using Full = std::integral_constant<bool, false>;
     auto chunk = [&](auto colmask, int k) -> bool {
         const int t0 = k * C;
         if constexpr (decltype(colmask)::value) {
             const int c = min(t0 + C + (lane & (C - 1)), nm1);
-            pB = bsym[c];
+            pB = load_b(c);
             pE = ld_relaxed(Ein + c);
         } else {   // an interior chunk: the next chunk's columns are inside [0, n)
             const uint32_t c = (uint32_t)(t0 + C + (lane & (C - 1)));
-            pB = bsym[c];
+            pB = load_b(c);
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C, std::integral_constant<bool, !decltype(colmask)::value>{});
-        band_steps<TM, C, ROWMASK, decltype(colmask)::value>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
+        band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
         Osave = O;
         if (!decltype(colmask)::value || k + 1 < nchunks) {   // an interior chunk is never the last
             IB = pB;
@@ -292,9 +367,17 @@ __device__ bool band_pass(const ChainArgs &A, const PairDesc &P, int g, int lane
     for (; k < nchunks; ++k)
         if (!chunk(Masked{}, k)) return false;
     publish(Osave, (nchunks - 1) * C, std::false_type{});
+    if constexpr (PK) {
 #pragma unroll
-    for (int k = 0; k < TM; ++k)
-        if (valid[k]) P.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];
+        for (int k = 0; k < TM; ++k)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                if (r0 + k < P.m_[q]) P.h_out[q][P.m_[q] - 1 - (r0 + k)] = (int32_t)((h[k] >> (16 * q)) & 0xffffu);
+    } else {
+#pragma unroll
+        for (int k = 0; k < TM; ++k)
+            if (valid[k]) P.h_out[m - 1 - (r0 + k)] = (int32_t)h[k];
+    }
     if (A.trace && t_id < A.trace_cap && lane == 0) A.trace[2 * (size_t)t_id + 1] = __builtin_amdgcn_s_memrealtime();
     return true;
 }
@@ -313,9 +396,26 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         const PairDesc P = A.pairs[task.x];
         const int g = task.y;
         const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
-        const bool ok = rowmask ? band_pass<TM, C, true>(A, P, g, lane, (int)t)
-                                : band_pass<TM, C, false>(A, P, g, lane, (int)t);
+        const bool ok = rowmask ? band_pass<TM, C, true, false>(A, P, g, lane, (int)t)
+                                : band_pass<TM, C, false, false>(A, P, g, lane, (int)t);
         if (!ok) break;
+    }
+}
+
+// The packed chain: ticket t runs band tasks[t].y of couple tasks[t].x.
+template <int TM, int C>
+__global__ __launch_bounds__(256) void comb_chain16_kernel(ChainArgs A)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    for (;;) {
+        unsigned t = 0;
+        if (lane == 0) t = atomicAdd(A.ticket, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if ((int)t >= A.ntasks) break;
+        if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) break;
+        const int2 task = A.tasks[t];
+        const CoupleDesc D = A.couples[task.x];
+        if (!band_pass<TM, C, false, true>(A, D, task.y, lane, (int)t)) break;
     }
 }
 
@@ -428,6 +528,7 @@ __device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, 
             // poll as band_pass does
             const unsigned long long t_start = __builtin_amdgcn_s_memrealtime();
             do {
+                if (A.poll_sleep) __builtin_amdgcn_s_sleep(2);
                 if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag)) || G->stop) return false;
                 if (__builtin_amdgcn_s_memrealtime() - t_start > A.timeout_ticks) {
                     G->stop = 1u;
@@ -618,6 +719,58 @@ __global__ void comb_prep_kernel(const PairDesc *pairs, unsigned *ctrl, unsigned
     }
 }
 
+// Seeds the rings of the packed chain (blockIdx.y = couple): column c of
+// slot 0 carries both pairs' top-edge strands m + c (0xffff past a pair's
+// n) under tag 0, slot 1 a tag no band waits for.  Block (0,0) clears the
+// ticket and abort words after folding the abort flag into `sticky`.
+__global__ void comb_prep16_kernel(const CoupleDesc *couples, unsigned *ctrl, unsigned *sticky)
+{
+    const CoupleDesc D = couples[blockIdx.y];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < D.n; c += stride) {
+        uint32_t v0 = 0;
+        for (int q = 0; q < 2; ++q) v0 |= (c < D.n_[q] ? (uint32_t)(D.m_[q] + c) : 0xffffu) << (16 * q);
+        D.E[c] = (unsigned long long)v0;                       // tag 0
+        D.E[(size_t)D.n + c] = 0xffffffff00000000ull;          // no band has this tag
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        if (sticky && ctrl[1]) *sticky = 1u;
+        ctrl[0] = 0u;
+        ctrl[1] = 0u;
+    }
+}
+
+// Both pairs' v_out from the last band's slot (blockIdx.y = couple).
+__global__ void comb_extract16_kernel(const CoupleDesc *couples)
+{
+    const CoupleDesc D = couples[blockIdx.y];
+    const unsigned long long *slot = D.E + (size_t)(D.nbands & 1) * (size_t)D.n;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < D.n; c += stride) {
+        const uint32_t w = (uint32_t)slot[c];
+        for (int q = 0; q < 2; ++q)
+            if (c < D.n_[q]) D.v_out[q][c] = (int32_t)((w >> (16 * q)) & 0xffffu);
+    }
+}
+
+// Clears n words (a kernel, so the launch sequence can be captured).
+__global__ void lcsk_zero_kernel(int32_t *x, int32_t n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
+// Range check of the packed chain's symbols (blockIdx.y = string).
+__global__ void check16_kernel(const SymbolRange *ranges, int32_t *bad)
+{
+    const SymbolRange R = ranges[blockIdx.y];
+    bool out = false;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < R.len;
+         i += (long long)gridDim.x * blockDim.x)
+        out = out || (uint32_t)R.x[i] > 0xfffdu;
+    if (__builtin_amdgcn_ballot_w64(out) != 0 && (threadIdx.x & 63) == 0) atomicOr(&bad[R.item], 1);
+}
+
 // v_out = the strands the last band left in its slot (blockIdx.y = pair).
 __global__ void comb_extract_kernel(const PairDesc *pairs)
 {
@@ -669,6 +822,12 @@ __global__ void strands_from_c2r_kernel(const int32_t *c2r, int32_t m, int32_t n
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v_out[i] = c2r[i];
     else if (i < (long long)n + m) h_out[i - n] = c2r[i];
+}
+
+__global__ void copy_kernel(int32_t *dst, const int32_t *src, long long count)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) dst[i] = src[i];
 }
 
 // Identity strands for empty grids (m == 0 or n == 0).
@@ -728,6 +887,41 @@ int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
+int lcsk_launch_check16(const SymbolRange *d_ranges, int32_t nranges, int32_t *d_bad, int32_t nitems,
+                        hipStream_t stream)
+{
+    if (nranges <= 0) return 0;
+    hipLaunchKernelGGL(lcsk_zero_kernel, dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0, stream, d_bad, nitems);
+    for (int32_t base = 0; base < nranges; base += 65535) {   // grid y is at most 65535
+        const int32_t cnt = std::min<int32_t>(65535, nranges - base);
+        hipLaunchKernelGGL(check16_kernel, dim3(4, (unsigned)cnt), dim3(256), 0, stream, d_ranges + base, d_bad);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_chain16(const ChainArgs &args, const CoupleDesc *d_couples, int ncouples, int max_n, int tile_m,
+                        int grid_blocks, int block_threads, hipStream_t stream)
+{
+    if (ncouples <= 0) return 0;
+    const int fill_blocks = (int)std::min<long long>(1024, std::max<long long>(1, ((long long)max_n + 255) / 256));
+    hipLaunchKernelGGL(comb_prep16_kernel, dim3(fill_blocks, ncouples), dim3(256), 0, stream, d_couples, args.ticket,
+                       args.sticky);
+    if (args.ntasks > 0) {
+        const dim3 grid(grid_blocks), block(block_threads);
+        switch (tile_m) {
+        case 1: comb_chain16_kernel<1, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        case 2: comb_chain16_kernel<2, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        case 4: comb_chain16_kernel<4, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        case 8: comb_chain16_kernel<8, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        case 16: comb_chain16_kernel<16, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        case 32: comb_chain16_kernel<32, kChainChunk><<<grid, block, 0, stream>>>(args); break;
+        default: return -22;
+        }
+    }
+    hipLaunchKernelGGL(comb_extract16_kernel, dim3(fill_blocks, ncouples), dim3(256), 0, stream, d_couples);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 int lcsk_launch_from_strands(const PairDesc *d_pairs, int npairs, int max_len, int32_t *perm,
                              const int64_t *d_perm_off, hipStream_t stream)
 {
@@ -755,6 +949,14 @@ int lcsk_launch_strands_from_perm(const int32_t *r2c, int32_t m, int32_t n, int3
     const dim3 grid((unsigned)((size + 255) / 256));
     hipLaunchKernelGGL(invert_perm_kernel, grid, dim3(256), 0, stream, r2c, (int32_t)size, scratch);
     hipLaunchKernelGGL(strands_from_c2r_kernel, grid, dim3(256), 0, stream, scratch, m, n, h_out, v_out);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_copy(int32_t *dst, const int32_t *src, long long count, hipStream_t stream)
+{
+    if (count <= 0) return 0;
+    const int blocks = (int)std::min<long long>(1024, (count + 255) / 256);
+    hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, stream, dst, src, count);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -127,6 +127,7 @@ int group_width(int variant)
     case LCS_VARIANT_AUTO:
     case LCS_VARIANT_CHAIN: return 1;
     case LCS_VARIANT_CHAIN_LDS: return 4;
+    case LCS_VARIANT_PACKED16: return 1;   // one pair per wave outside batches
     default: return 0;
     }
 }
@@ -198,8 +199,69 @@ int build_launch(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, un
 void rebase_launch(PreparedLaunch &L, const void *device_blob)
 {
     const uintptr_t b = (uintptr_t)device_blob;
-    L.args.pairs = (const PairDesc *)(b + (uintptr_t)L.args.pairs);
+    if (L.packed) L.args.couples = (const CoupleDesc *)(b + (uintptr_t)L.args.couples);
+    else L.args.pairs = (const PairDesc *)(b + (uintptr_t)L.args.pairs);
     L.args.tasks = (const int2 *)(b + (uintptr_t)L.args.tasks);
+}
+
+// Ring words of a packed launch: one ring of lcsk_ring_words(max n) per
+// couple, from the pairs' own extents (D.n is only set by build_launch16).
+size_t couples_ring_words(const std::vector<CoupleDesc> &couples)
+{
+    size_t w = 0;
+    for (const auto &D : couples) w += lcsk_ring_words(std::max(D.n_[0], D.n_[1]));
+    return w;
+}
+
+// One packed-chain launch over `couples` (one tile height): rings carved
+// from `ring`, tasks band-major over the couples.
+int build_launch16(lcs_ctx *ctx, std::vector<CoupleDesc> &couples, int tm, int wps, unsigned long long *ring,
+                   unsigned *ctrl, std::vector<unsigned char> &blob, PreparedLaunch &L)
+{
+    if (!valid_tile_m(tm) || wps < 1 || wps > 8) return LCS_EINVAL;
+    const int nc = (int)couples.size();
+    L = PreparedLaunch{};
+    L.packed = true;
+    if (nc == 0) return LCS_OK;
+    int32_t max_bands = 0, max_n = 0;
+    unsigned long long *E = ring;
+    for (auto &D : couples) {
+        D.tile_m = tm;
+        D.m = std::max(D.m_[0], D.m_[1]);
+        D.n = std::max(D.n_[0], D.n_[1]);
+        D.nbands = bands_of(D.m, D.n, tm);
+        D.E = E;
+        E += lcsk_ring_words(D.n);
+        max_bands = std::max(max_bands, D.nbands);
+        max_n = std::max(max_n, std::max(D.n, D.m));
+    }
+    std::vector<int2> tasks;
+    for (int32_t g = 0; g < max_bands; ++g)
+        for (int c = 0; c < nc; ++c)
+            if (g < couples[(size_t)c].nbands) tasks.push_back(make_int2(c, g));
+    const size_t cbytes = sizeof(CoupleDesc) * (size_t)nc, task_bytes = sizeof(int2) * tasks.size();
+    const size_t base = (blob.size() + 15) & ~(size_t)15;
+    blob.resize(base + cbytes + task_bytes);
+    std::memcpy(blob.data() + base, couples.data(), cbytes);
+    if (task_bytes) std::memcpy(blob.data() + base + cbytes, tasks.data(), task_bytes);
+    ChainArgs &A = L.args;
+    A.couples = (const CoupleDesc *)(uintptr_t)base;
+    A.tasks = (const int2 *)(uintptr_t)(base + cbytes);
+    A.ntasks = (int32_t)tasks.size();
+    A.ticket = ctrl;
+    A.abort_flag = ctrl + 1;
+    A.sticky = ctx->ctrl + kStickyWord;
+    A.timeout_ticks = ctx->timeout_ticks;
+    A.trace = ctx->band_trace;
+    A.trace_cap = ctx->band_trace_cap;
+    A.group_w = 1;
+    const long long waves = std::min<long long>((long long)tasks.size(), 4LL * ctx->num_cus * wps);
+    L.blocks = (int)std::max<long long>(1, (waves + 3) / 4);
+    L.threads = 256;
+    L.npairs = nc;
+    L.max_n = max_n;
+    L.tile_m = tm;
+    return LCS_OK;
 }
 
 // Prepares one chain launch over `pairs` (one tile height): assigns each
@@ -228,6 +290,8 @@ int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, 
 int launch_prepared(const PreparedLaunch &L, hipStream_t stream)
 {
     if (L.npairs == 0) return LCS_OK;
+    if (L.packed)
+        return lcsk_launch_chain16(L.args, L.args.couples, L.npairs, L.max_n, L.tile_m, L.blocks, L.threads, stream);
     return lcsk_launch_chain(L.args, L.args.pairs, L.npairs, L.max_n, L.tile_m, L.blocks, L.threads, stream);
 }
 
@@ -241,19 +305,31 @@ int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, h
     if ((rc = prepare_pairs(ctx, pairs, tm, wps, (unsigned long long *)ctx->ring.ptr, ctx->descs, ctx->ctrl, L,
                             stream, group_w)))
         return rc;
-    return launch_prepared(L, stream);
+    if (hipEventRecord(ctx->evc0, stream) != hipSuccess) return LCS_EIO;
+    if ((rc = launch_prepared(L, stream))) return rc;
+    if (hipEventRecord(ctx->evc1, stream) != hipSuccess) return LCS_EIO;
+    ctx->chain_timed = true;
+    return LCS_OK;
 }
 
-// Strips a single pair is cut into when its bands alone would leave SIMDs
-// idle: the merge tree of the steady ant joins the strips (k = m each).
-// By default only pairs with m <= kDefaultStripMaxM are cut (merges of
-// larger cores run over the whole GPU and are chosen explicitly).
+// Strips a single pair is cut into.  A thin pair (m <= kDefaultStripMaxM)
+// whose bands alone would leave SIMDs idle is cut until the strips' bands
+// fill them.  A tall pair whose bands already fill every SIMD is cut in two
+// when it is wide: the persistent waves then comb each strip's lower bands
+// as a second round, so the chain's fill (one band-to-band lag per band
+// before the last band starts) is paid for half the bands, at the price of
+// one whole-GPU steady-ant merge with k = m.  At m = n = 2^20 two strips
+// took 141.3 ms against 146.3 ms for one (the same kernels), four 143.6 ms
+// and eight 154.4 ms (more merge levels).
 constexpr int32_t kDefaultStripMaxM = 32768;
+constexpr int32_t kTallStripMinN = 1 << 18;
 int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 {
-    if (m > kDefaultStripMaxM || n < 2 * 8192) return 1;
     const long long simds = 4LL * ctx->num_cus;
     const long long bands = bands_of(m, n, tm);
+    if (m > kDefaultStripMaxM)
+        return (m <= kMaxCoreK && n >= kTallStripMinN && bands >= simds) ? 2 : 1;
+    if (n < 2 * 8192) return 1;
     int s = 1;
     while ((long long)s * bands < simds && s < 64 && (long long)n / (2 * s) >= 8192) s *= 2;
     return s;
@@ -471,6 +547,7 @@ int lcs_ctx_create(int device, size_t hbm_budget, lcs_ctx **out)
     ctx->budget = hbm_budget ? hbm_budget : free_b / 4 * 3;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return LCS_EIO; }
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreate(&ctx->evc0) != hipSuccess || hipEventCreate(&ctx->evc1) != hipSuccess ||
         ctx->staging.init() != LCS_OK ||
         hipEventCreateWithFlags(&ctx->ws_done, hipEventDisableTiming) != hipSuccess) {
         lcs_ctx_destroy(ctx);
@@ -504,6 +581,8 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     if (ctx->band_trace) (void)hipFree(ctx->band_trace);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->evc0) (void)hipEventDestroy(ctx->evc0);
+    if (ctx->evc1) (void)hipEventDestroy(ctx->evc1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LCS_OK;
@@ -559,6 +638,18 @@ int lcs_ctx_info(lcs_ctx *ctx, int32_t *num_cus, double *last_kernel_ms)
     }
     if (num_cus) *num_cus = ctx->num_cus;
     if (last_kernel_ms) *last_kernel_ms = ctx->last_kernel_ms;
+    return LCS_OK;
+}
+
+int lcs_ctx_chain_ms(lcs_ctx *ctx, double *chain_ms)
+{
+    if (!ctx || !chain_ms) return LCS_EINVAL;
+    if (!ctx->chain_timed) return LCS_ENOENT;
+    if (hipSetDevice(ctx->device) != hipSuccess) return LCS_EIO;
+    if (hipEventSynchronize(ctx->evc1) != hipSuccess) return LCS_EIO;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->evc0, ctx->evc1) != hipSuccess) return LCS_EIO;
+    *chain_ms = ms;
     return LCS_OK;
 }
 

@@ -39,6 +39,7 @@ struct Staging {
 struct PreparedLaunch {
     ChainArgs args{};
     int blocks = 0, threads = 0, npairs = 0, max_n = 0, tile_m = 0;
+    bool packed = false;   // args.couples (the 16-bit packed chain), npairs couples
 };
 
 // Schedule selection state of one mode (LCS_TUNE_SINGLE / LCS_TUNE_BATCH):
@@ -73,6 +74,10 @@ struct lcs_ctx {
     size_t held = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // around the band-chain launch (seed, chain, extract) of the last
+    // single-pair call: its time alone, without merges or conversions
+    hipEvent_t evc0 = nullptr, evc1 = nullptr;
+    bool chain_timed = false;
     unsigned *ctrl = nullptr;            // [0] ticket, [1] abort flag, rest spare
     unsigned long long timeout_ticks = 500000000ull;  // 5 s at 100 MHz
     double last_kernel_ms = 0.0;
@@ -110,6 +115,10 @@ size_t pairs_ring_words(const std::vector<PairDesc> &pairs);
 int build_launch(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
                  unsigned *ctrl, int group_w, std::vector<unsigned char> &blob, PreparedLaunch &L);
 void rebase_launch(PreparedLaunch &L, const void *device_blob);
+// The same for couples of pairs combed by the 16-bit packed chain.
+int build_launch16(lcs_ctx *ctx, std::vector<CoupleDesc> &couples, int tm, int wps, unsigned long long *ring,
+                   unsigned *ctrl, std::vector<unsigned char> &blob, PreparedLaunch &L);
+size_t couples_ring_words(const std::vector<CoupleDesc> &couples);
 // group_w: 1 = register chain (one wave per band), 4 = LDS groups of 4 bands
 int prepare_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, unsigned long long *ring,
                   DevBuf &descs, unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w = 1);

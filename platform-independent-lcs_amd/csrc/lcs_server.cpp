@@ -229,6 +229,7 @@ int launch_slot(lcs_server *S, Slot &slot)
     for (int c = 0; c < kClasses; ++c) {
         if (groups[(size_t)c].empty()) continue;
         rebase_launch(launches[c], slot.descs.ptr);
+        launches[c].args.poll_sleep = 1;   // slots and classes overlap on the SIMDs
         if ((rc = launch_prepared(launches[c], slot.stream))) return rc;
     }
     return hipEventRecord(slot.done, slot.stream) == hipSuccess ? LCS_OK : LCS_EIO;

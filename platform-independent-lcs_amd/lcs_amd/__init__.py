@@ -27,7 +27,7 @@ ERRORS = {
     -34: "ERANGE", -110: "ETIMEDOUT", -125: "ECANCELED",
 }
 
-VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS = 0, 1, 2
+VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS, VARIANT_PACKED16 = 0, 1, 2, 3
 TUNE_SINGLE, TUNE_BATCH = 0, 1
 SCHED_FROM = {0: "default", 1: "table", 2: "model", 3: "autotune", 4: "caller"}
 SEMI_STRING_SUBSTRING, SEMI_SUBSTRING_STRING, SEMI_PREFIX_SUFFIX, SEMI_SUFFIX_PREFIX, SEMI_DOMINANCE = 0, 1, 2, 3, 4
@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
     L.lcs_ctx_device_name.argtypes = [vp, ctypes.c_char_p, sz]
     L.lcs_ctx_info.argtypes = [vp, _i32p, ctypes.POINTER(ctypes.c_double)]
     L.lcs_ctx_status.argtypes = [vp]
+    L.lcs_ctx_chain_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.lcs_ctx_set_timeout.argtypes = [vp, ctypes.c_uint64]
     L.lcs_ctx_memory.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     L.lcs_comb.argtypes = [vp, _i32p, i32, _i32p, i32, sched_p, _i32p, _i32p, _i32p]
@@ -195,6 +196,12 @@ class Context:
         ms = ctypes.c_double()
         _check("lcs_ctx_info", lib().lcs_ctx_info(self._h, ctypes.byref(cus), ctypes.byref(ms)))
         return cus.value, ms.value
+
+    def chain_ms(self) -> float:
+        """lcs_ctx_chain_ms: the band-chain launch alone in the last single-pair call."""
+        ms = ctypes.c_double()
+        _check("lcs_ctx_chain_ms", lib().lcs_ctx_chain_ms(self._h, ctypes.byref(ms)))
+        return ms.value
 
     def status(self) -> int:
         """lcs_ctx_status: waits for the context's kernels; LCS_OK or LCS_ETIMEDOUT."""

@@ -154,16 +154,18 @@ def test_thin_pair_eight_row_strips(ctx):
 
 
 def test_thin_pair_at_the_largest_strip_height(ctx):
-    # m = 32768, the largest height that is cut into strips: the merges run
-    # on the global-memory steady-ant core; one row more and the pair is
-    # combed as one strip
+    # m = 32768, the largest height the default schedule cuts into strips;
+    # one row more and the default combs one strip, while explicit strips
+    # merge on the whole-GPU steady ant (k = m = 32769) to the same strands
     a = lcs_amd.gen_zipf(13, 1 << 15, 4, 1.1)
     b = lcs_amd.gen_planted(14, a, 1 << 18, 4, 1.1, 4, 0.05)
     h, v, llcs = ctx.comb(a, b)
     h1, v1, llcs1 = ctx.comb(a, b, lcs_amd.schedule(strips=1))
     assert np.array_equal(h, h1) and np.array_equal(v, v1) and llcs == llcs1
-    with pytest.raises(lcs_amd.LcsError):
-        ctx.comb(np.append(a, 1).astype(np.int32), b, lcs_amd.schedule(strips=2))
+    a2 = np.append(a, 1).astype(np.int32)
+    h2, v2, _ = ctx.comb(a2, b, lcs_amd.schedule(strips=2))
+    h3, v3, _ = ctx.comb(a2, b, lcs_amd.schedule(strips=1))
+    assert np.array_equal(h2, h3) and np.array_equal(v2, v3)
 
 
 LDS = 2  # LCS_VARIANT_CHAIN_LDS
