@@ -332,6 +332,51 @@ def bench_tiled_merge(ctx, da, db, m: int, n: int, h_ref, v_ref, strips=(1, 2, 4
     return out
 
 
+def bench_config1(ctx):
+    """The reference's own CPU-runnable case: m = n = 4096 uniform over
+    alphabet 4.  GPU: lcs_comb_device (best of 5 after a warm-up); CPU: the
+    oracle's sequential comb (lcs_reference.hpp:5-69) and its vectorised
+    tiled comb on the whole pair; all three outputs compared bit for bit."""
+    import oracle  # test infrastructure: the CPU side of this leg only
+    L = lcs_amd.lib()
+    w = lcs_amd.workload_single("uniform4096", seed=1)
+    m, n = w.a.size, w.b.size
+    da, db = H.DeviceArray.from_numpy(w.a), H.DeviceArray.from_numpy(w.b)
+    dh, dv = H.DeviceArray(m), H.DeviceArray(n)
+    ks, walls = [], []
+    for i in range(6):
+        t0 = time.perf_counter()
+        rc = L.lcs_comb_device(ctx.handle, da.ptr, m, db.ptr, n, None, dh.ptr, dv.ptr, None)
+        if rc:
+            raise lcs_amd.LcsError("lcs_comb_device", rc)
+        H.synchronize()
+        st = ctx.status()
+        if st:
+            raise lcs_amd.LcsError("lcs_ctx_status", st)
+        if i:
+            walls.append(time.perf_counter() - t0)
+            ks.append(ctx.info()[1])
+    h, v = dh.numpy(), dv.numpy()
+    t0 = time.perf_counter()
+    h1, v1 = oracle.comb(w.a, w.b, order="rows")
+    seq_s = time.perf_counter() - t0
+    threads = host_cpus()[0]
+    t0 = time.perf_counter()
+    h2, v2 = oracle.comb(w.a, w.b, order="simd", threads=threads)
+    simd_s = time.perf_counter() - t0
+    for d in (da, db, dh, dv):
+        d.free()
+    cells = float(m) * n
+    return {"workload": "m=n=4096 uniform alphabet 4 (the reference's CPU-runnable case)",
+            "gpu_ms": round(min(walls) * 1e3, 3), "gpu_kernel_ms": round(min(ks), 3),
+            "gpu_gcups": round(cells / min(walls) / 1e9, 1),
+            "cpu_sequential_ms": round(seq_s * 1e3, 1), "cpu_sequential_gcups": round(cells / seq_s / 1e9, 3),
+            "cpu_vectorised_ms": round(simd_s * 1e3, 2), "cpu_vectorised_threads": threads,
+            "bit_exact": bool(np.array_equal(h, h1) and np.array_equal(v, v1) and np.array_equal(h, h2)
+                              and np.array_equal(v, v2)),
+            "llcs": int((v < m).sum())}
+
+
 def bench_semilocal(ctx, r2c, m: int, n: int, llcs: int, nq: int = 1 << 22):
     """Semi-local read-out on the permutation of the m=n=2^20 pair: build the
     dominance index from the device permutation, then answer nq random
@@ -577,6 +622,8 @@ def main():
     }
     if checked is not None:
         out["cross_schedule_bit_exact"] = checked
+    if not args.no_cpu:
+        out["config1"] = bench_config1(ctx)
     if not args.no_merge:
         out["tiled_merge"] = bench_tiled_merge(ctx, da, db, m, n, h, v)
     if not args.no_semilocal:

@@ -1038,10 +1038,21 @@ using namespace lcsb;
 #ifndef LCS_BIG_LEAF_L
 #define LCS_BIG_LEAF_L 11
 #endif
+// cores of 2^16 points and more take leaves of up to 2^LCS_BIG_LEAF_L_LARGE
+#ifndef LCS_BIG_LEAF_L_LARGE
+#define LCS_BIG_LEAF_L_LARGE 11
+#endif
 constexpr int kBigMinL = LCS_BIG_MIN_L;
 constexpr int kBigLeafMaxL = LCS_BIG_LEAF_L;
-static_assert(kBigMinL >= 11 && kBigLeafMaxL >= 10 && kBigLeafMaxL <= 14, "big-core leaf sizes");
-static int big_leaf_L(int L) { return L - 1 < kBigLeafMaxL ? L - 1 : kBigLeafMaxL; }
+constexpr int kBigLeafLargeL = LCS_BIG_LEAF_L_LARGE;
+static_assert(kBigMinL >= 11 && kBigLeafMaxL >= 10 && kBigLeafMaxL <= 14 && kBigLeafLargeL >= 10 &&
+                  kBigLeafLargeL <= 14,
+              "big-core leaf sizes");
+static int big_leaf_L(int L)
+{
+    const int lim = L >= 16 ? kBigLeafLargeL : kBigLeafMaxL;
+    return L - 1 < lim ? L - 1 : lim;
+}
 static bool is_big(int L) { return L >= kBigMinL; }
 
 size_t lcsb_core_scratch_words(int32_t k)
