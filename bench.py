@@ -605,7 +605,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded mt19937_64: uniform alphabet-4 A, B = A + 10% subs + 2% indels)",
         "config": {"workload": f"single pair m=n=2^{args.log2n}, tiled band-chain comb + steady-ant merge "
-                               "(library default schedule: two column strips, one merge with k = m)",
+                               "(library default schedule: four column strips, three merges with k = m)",
                    "m": m, "n": n, "alphabet": 4, "tile_m": args.tile_m or None, "variant": args.variant or None,
                    "llcs": llcs},
         "roofline": {"bound": "valu", "achieved": round(achieved_ops / 1e12, 3), "peak": round(peak_ops / 1e12, 3),

@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Largest padded core (2^L): cores up to 2^14 run in one workgroup's LDS,
+// Largest padded core (2^L): cores up to 2^11 run in one workgroup's LDS,
 // larger ones over the whole GPU (their grid of ant-walk boxes is sized
 // for nodes of at most 2^22 points).
 constexpr int kMaxCoreL = 22;
@@ -14,7 +14,7 @@ struct BraidCoreDesc {
     const int32_t *Q;   // k: shared row -> column
     int32_t *R;         // k: row -> column of the sticky product
     int32_t k, L;       // padded size 2^L >= k
-    int32_t *scratch;   // lcsb_core_scratch_words(k) words (k > 2^14: the big-core workspace)
+    int32_t *scratch;   // lcsb_core_scratch_words(k) words (the big-core workspace; none for k <= 2^11)
 };
 
 // Core descriptors passed by value in the kernel arguments, one per block.

@@ -23,9 +23,10 @@
 //    boundary with a single pass over the node's columns; then one thread
 //    per chunk walks it with the reference's incremental updates and places
 //    the good points.
-// The union of the children's points and the per-column view live in LDS
-// (16-bit entries with the lo/hi flag in bit 15), so the walks run at LDS
-// latency.  Cores up to K = 32768 take this path.
+// The union of the children's points, the per-column view and the parents
+// of every depth's partition live in LDS (16-bit entries, the lo/hi flag in
+// bit 15), so the walks and the lifts run at LDS latency.  Cores up to
+// K = 2048 take this path (the parents alone are 4 K log2(K) bytes).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -38,15 +39,28 @@
 namespace lcsb {
 
 constexpr int kThreads = 1024;
-constexpr int kChunkRows = 32;
+// rows per walked chunk of a node (nodes above 128 chunks take s / 128)
+#ifndef LCS_CORE_CHUNK_ROWS
+#define LCS_CORE_CHUNK_ROWS 8
+#endif
+constexpr int kChunkRows = LCS_CORE_CHUNK_ROWS;
+static_assert(kChunkRows >= 4 && kChunkRows <= 64 && (kChunkRows & (kChunkRows - 1)) == 0, "chunk rows");
 constexpr uint16_t kLoFlag = 0x8000u;
 constexpr uint16_t kIdx = 0x7fffu;
-// cores up to this size keep keys and products in LDS (braid_core_lds_kernel)
-constexpr int kLdsCoreK = 16384;
-// row chunks per node are at most 128 (chunk height max(32, s/128)); each wave
+// cores up to this size keep keys, products and parents in LDS
+// (braid_core_lds_kernel and the leaves of larger cores)
+constexpr int kLdsCoreL = 11;
+constexpr int kLdsCoreK = 1 << kLdsCoreL;
+// row chunks per node are at most 128 (chunk height max(kChunkRows, s/128)); each wave
 // keeps 2 x kMaxChunks bucket counters in LDS for the boundary sweeps
 constexpr int kMaxChunks = 128;
-constexpr size_t lds_core_bytes(int K) { return (size_t)4 * K * sizeof(uint16_t) + ((size_t)K / 64 + 1) * 4; }
+// dynamic LDS of a core of 2^L points: keys / U, CI, R, Rn (4K uint16), the
+// K/64+1 group counts, then the parents of P and of Q at each depth (2 L K uint16)
+constexpr size_t lds_core_bytes(int L)
+{
+    return (size_t)4 * (1 << L) * sizeof(uint16_t) + ((size_t)(1 << L) / 64 + 1) * 4 +
+           (size_t)2 * L * (1 << L) * sizeof(uint16_t);
+}
 
 // Inclusive prefix sum over the 64 lanes of a wave with DPP: four row_shr
 // steps scan each row of 16 lanes, then row_bcast:15 and row_bcast:31 carry
@@ -66,12 +80,12 @@ __device__ __forceinline__ int wave_incl_scan(int x)
 // products into node coordinates (the reference's inverse_mapping, :191-202)
 // and run the chunked ant passage (:123-189) for every node of the depth.
 // U, CI are in LDS; R (children's products) and Rn (this depth's) are
-// either global int32 or LDS uint16 arrays; pp, pq map depth d+1 positions
-// to depth d positions; bT, bD hold K/32 chunk entries.
+// either global int32 or LDS uint16 arrays; pp, pq (LDS) map depth d+1
+// positions to depth d positions; bT, bD hold K/kChunkRows chunk entries.
 template <typename UT, typename RT, typename RNT>
-__device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT *Rn, const int32_t *__restrict__ pp,
-                                                const int32_t *__restrict__ pq, int K, int s, int32_t *bT,
-                                                int32_t *bD, int *hist)
+__device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT *Rn, const uint16_t *pp,
+                                                const uint16_t *pq, int K, int s, int32_t *bT, int32_t *bD,
+                                                int *hist)
 {
     const int h = s >> 1;
     // lift the children's products into node coordinates
@@ -88,16 +102,17 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
         Rn[p_pos] = lc;
     }
     __syncthreads();
-    // chunk height: 32 rows, or s/128 for big nodes, balancing the
+    // chunk height: kChunkRows, or s/128 for big nodes, balancing the
     // boundary scans (O(s) each) against the walks (O(rows) each)
-    const int rc = s < kChunkRows ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
+    // (nodes under 64 rows are one chunk: the boundary sweeps read 64 columns per round)
+    const int rc = s < 64 ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
     const int rc_log = __builtin_ctz(rc);   // rc and s are powers of two
     const int nch = s / rc;            // chunks per node
     const int total = K / rc;          // chunks at this depth
     // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc).
     // A node's first chunk (j = 0) enters at T = 0; its entry is computed in
-    // the walk, so only chunks j > 0 (nodes of 64 rows or more, at most K/32
-    // chunks) store entries in bT, bD.  For chunk j > 0 entering at row
+    // the walk, so only chunks j > 0 (nodes of 64 rows or more, at most
+    // K/kChunkRows chunks) store entries in bT, bD.  For chunk j > 0 entering at row
     // Dr = s - j rc:
     //   T(Dr) = min{ c : Z(c) >= Y }, Z(c) = #{c' < c : z(c')},
     //   z(c) = hi(c) ? row(c) < Dr : row(c) >= Dr,  Y = #{lo rows >= Dr},
@@ -293,7 +308,7 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
 // One depth of the top-down split with the keys in LDS: a stable partition
 // of every aligned segment of size s by bit `bit` (zeros first), the ranks
 // from wave ballots and one scan of the 64-key group counts.
-// par[pos] = old position (global).  K >= 64.  A wave takes the groups
+// par[pos] = old position (LDS).  K >= 64.  A wave takes the groups
 // wave, wave + 16, ...; U of them at a time, with every load of the U groups
 // issued before the first ballot, so their LDS latencies overlap (the
 // compiler does not unroll a loop around a ballot by itself).
@@ -311,7 +326,7 @@ __device__ __forceinline__ void split_count_groups(const uint16_t *key, int bit,
 }
 
 template <int U>
-__device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t *key_new, int32_t *par, int s,
+__device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t *key_new, uint16_t *par, int s,
                                                    int bit, const int32_t *gpre, int g0, int lane)
 {
     const unsigned long long lt = (1ull << lane) - 1;
@@ -345,11 +360,11 @@ __device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t
         }
         const int pos = b ? seg + (s - ones) + before : seg + (x - seg - before);
         key_new[pos] = kv[u];
-        par[pos] = x;
+        par[pos] = (uint16_t)x;
     }
 }
 
-__device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t *par, int K, int s, int bit,
+__device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, uint16_t *par, int K, int s, int bit,
                                 int32_t *gpre)
 {
     constexpr int kW = kThreads / 64;
@@ -376,23 +391,23 @@ __device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, int32_t 
     __syncthreads();
 }
 
-// The sticky product with keys, both products and the chunk entries in LDS
-// for cores up to K = kLdsCoreK, one workgroup per core.  LDS: 4K uint16
-// (keys / U, CI, R, Rn) plus K/64+1 group counts.  P holds row -> shared
-// column; Q holds shared row -> column, or (QKEY) column -> shared row,
-// the keys themselves.  Values are read modulo K (a leaf of a big core
-// reads its slice of the big core's keys).
+// The sticky product with keys, both products, the parents of every depth
+// and the chunk entries in LDS for cores up to K = kLdsCoreK, one workgroup
+// per core; the dynamic LDS is laid out as lds_core_bytes(L) says.  P holds
+// row -> shared column; Q holds shared row -> column, or (QKEY) column ->
+// shared row, the keys themselves.  Values are read modulo K (a leaf of a
+// big core reads its slice of the big core's keys).
 template <bool QKEY>
 __device__ __forceinline__ void core_lds_body(const int32_t *P, const int32_t *Q, int32_t *Rout, int k, int L,
-                                              int32_t *scratch, uint16_t *lds, int32_t *bT, int32_t *bD, int *hist)
+                                              uint16_t *lds, int32_t *bT, int32_t *bD, int *hist)
 {
     const int K = 1 << L, mask = K - 1;
     uint16_t *key = lds, *key2 = lds + K;             // top-down (P, then Q)
     uint16_t *U = lds, *CI = lds + K;                 // bottom-up (reuses the key space)
     uint16_t *R = lds + 2 * (size_t)K, *Rn = lds + 3 * (size_t)K;
     int32_t *gpre = (int32_t *)(lds + 4 * (size_t)K);
-    int32_t *parP = scratch;                          // L * K
-    int32_t *parQ = parP + (size_t)L * K;             // L * K
+    uint16_t *parP = lds + 4 * (size_t)K + 2 * ((size_t)K / 64 + 1);   // L * K
+    uint16_t *parQ = parP + (size_t)L * K;                              // L * K
 
     // P ordered by row (key = shared column)
     for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(x < k ? (P[x] & mask) : x);
@@ -432,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
     extern __shared__ uint16_t lds[];
     __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
     __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
-    core_lds_body<false>(D.P, D.Q, D.R, D.k, D.L, D.scratch, lds, bT, bD, hist);
+    core_lds_body<false>(D.P, D.Q, D.R, D.k, D.L, lds, bT, bD, hist);
 }
 
 // ---------------------------------------------------------------- big cores
@@ -466,8 +481,13 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 // indices with the lo flag in bit 31.
 constexpr int kBigBatch = 16;          // products per big-core launch
 constexpr int kSplitTile = 2048;       // elements per block of a device-wide partition
-constexpr int kHistSeg = 8192;         // entries per block of the boundary histograms
 constexpr int kMaxLines = 8192;        // grid lines per node and direction
+// box side g of a node's grid (nodes above kMaxLines * g take g = s / kMaxLines)
+#ifndef LCS_BIG_BOX_G
+#define LCS_BIG_BOX_G 64
+#endif
+constexpr int kBoxG = LCS_BIG_BOX_G;
+static_assert(kBoxG >= 16 && kBoxG <= 512 && (kBoxG & (kBoxG - 1)) == 0, "box side");
 constexpr uint32_t kLo31 = 0x80000000u;
 
 struct BigCore {
@@ -490,16 +510,22 @@ __host__ __device__ inline BigDepth big_depth(int L, int d)
 {
     BigDepth D;
     D.s = 1 << (L - d);
-    D.nb = D.s / 64 < kMaxLines ? D.s / 64 : kMaxLines;
+    D.nb = D.s / kBoxG < kMaxLines ? D.s / kBoxG : kMaxLines;
     D.glog = __builtin_ctz((unsigned)(D.s / D.nb));
-    D.w = D.s < kHistSeg ? D.s : kHistSeg;
+    // Histogram blocks of w entries: every open boundary sweeps (on average)
+    // half a block, nb w / 2 entries per node and direction, while the
+    // blocks' prefixes cost about 2 (s / w)(nb + 1) words written and read;
+    // w = 2 sqrt(s) balances the two (clamped to [512, 8192], at most s).
+    const int e = L - d, wl = 1 + (e + 1) / 2;
+    D.w = 1 << (wl < 9 ? 9 : (wl > 13 ? 13 : wl));
+    if (D.w > D.s) D.w = D.s;
     D.nsegs = D.s / D.w;
     return D;
 }
 
 // word offsets inside one product's workspace
 struct BigLayout {
-    size_t keyP, keyQ, keyP2, keyQ2, parP, parQ, leaf, R, Rn, U, CI, cnt, segtot, hist, info, T, total;
+    size_t keyP, keyQ, keyP2, keyQ2, parP, parQ, R, Rn, U, CI, cnt, segtot, hist, T, total;
 };
 
 __host__ __device__ inline BigLayout big_layout(int L, int L0)
@@ -514,7 +540,6 @@ __host__ __device__ inline BigLayout big_layout(int L, int L0)
     B.keyQ2 = o; o += K;
     B.parP = o; o += (size_t)d0 * K;
     B.parQ = o; o += (size_t)d0 * K;
-    B.leaf = o; o += (size_t)2 * L0 * K;     // per leaf: L0 * 2^L0 parents of P and of Q
     B.R = o; o += K;
     B.Rn = o; o += K;
     B.U = o; o += K;
@@ -530,7 +555,6 @@ __host__ __device__ inline BigLayout big_layout(int L, int L0)
         lines = nodes * (size_t)D.nb > lines ? nodes * (size_t)D.nb : lines;
     }
     B.hist = o; o += hist;
-    B.info = o; o += 2 * 2 * lines;          // (segment, rank) per boundary
     B.T = o; o += 2 * lines;
     B.total = o;
     return B;
@@ -654,8 +678,7 @@ __global__ __launch_bounds__(kThreads) void big_leaf_kernel(BigBatch B)
     __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
     __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
     // depth d0 of the big core's own recursion: the products go to R
-    core_lds_body<true>(keyP, keyQ, C.ws + Y.R + v0, s0, B.L0, C.ws + Y.leaf + (size_t)blockIdx.x * 2 * B.L0 * s0,
-                        lds, bT, bD, hist);
+    core_lds_body<true>(keyP, keyQ, C.ws + Y.R + v0, s0, B.L0, lds, bT, bD, hist);
 }
 
 // lift the children's products into node coordinates (node size s).  The
@@ -704,7 +727,7 @@ __global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
         atomicAdd(&h[bk], (e & kLo31) ? 1u : 0x10000u);
     }
     __syncthreads();
-    const int per = D.nb / 256 > 0 ? D.nb / 256 : 1;   // nb is a power of two >= 64
+    const int per = D.nb / 256 > 0 ? D.nb / 256 : 1;   // nb is a power of two
     const int lo = min((int)threadIdx.x * per, D.nb), hi = min(lo + per, D.nb);
     uint32_t run = 0;
     for (int j = lo; j < hi; ++j) run += h[j];
@@ -725,47 +748,15 @@ __global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
     if (threadIdx.x == 255) out[D.nb] = sums[255];
 }
 
-// One thread per (y, node, boundary j): Y_j from the blocks' lo prefixes,
-// then the block where the running count reaches Y_j and the rank still
-// needed inside it.  Boundaries with Y_j = 0 (or j = 0) get T = 0 here.
-__global__ __launch_bounds__(256) void big_bound_kernel(BigBatch B, int d)
-{
-    const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
-    const BigDepth D = big_depth(B.L, d);
-    const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nodes * D.nb) return;
-    const int node = t / D.nb, j = t % D.nb;
-    const size_t stride = (size_t)D.nb + 1;
-    const uint32_t *H = (const uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + (size_t)node * D.nsegs) * stride;
-    int32_t *info = C.ws + Y.info + 2 * ((size_t)y * nodes * D.nb + t);
-    int32_t *T = C.ws + Y.T + (size_t)y * nodes * D.nb + t;
-    int Yj = 0;
-    if (j > 0)
-        for (int g = 0; g < D.nsegs; ++g) Yj += (int)(H[g * stride + j] & 0xffffu);
-    if (j == 0 || Yj == 0) {
-        info[0] = -1;
-        info[1] = 0;
-        *T = 0;
-        return;
-    }
-    int Z = 0, seg = D.nsegs - 1;
-    for (int g = 0; g < D.nsegs; ++g) {
-        const uint32_t P = H[g * stride + j], tot = H[g * stride + D.nb];
-        const int c = (int)(P & 0xffffu) + (int)(tot >> 16) - (int)(P >> 16);
-        if (Z + c >= Yj) {
-            seg = g;
-            break;
-        }
-        Z += c;
-    }
-    info[0] = seg;
-    info[1] = Yj - Z;
-}
-
-// One wave per (y, node, boundary) left open: sweep its block for the
-// entry where the count reaches the rank; T = that entry's position + 1.
+// One wave per (y, node, boundary j): Y_j from the blocks' lo prefixes,
+// the block where the running count reaches Y_j (a wave scan over the
+// blocks, 64 at a time), then a sweep of that block for the entry where the
+// count reaches the rank still needed; T = that entry's position + 1.
+// Boundaries with j = 0 or Y_j = 0 get T = 0.  The block (w >= 512 entries,
+// a power of two) is read kSweepRounds x 64 entries at a time, all loads
+// issued before the first ballot, so one memory latency covers that many
+// rounds of the scan.
+constexpr int kSweepRounds = 8;
 __global__ __launch_bounds__(256) void big_sweep_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
@@ -774,43 +765,95 @@ __global__ __launch_bounds__(256) void big_sweep_kernel(BigBatch B, int d)
     const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= nodes * D.nb) return;
-    const int32_t *info = C.ws + Y.info + 2 * ((size_t)y * nodes * D.nb + t);
-    const int seg = info[0];
-    if (seg < 0) return;
-    int need = info[1];
     const int node = t / D.nb, j = t % D.nb;
+    int32_t *T = C.ws + Y.T + (size_t)y * nodes * D.nb + t;
+    if (j == 0) {
+        if (lane == 0) *T = 0;
+        return;
+    }
+    const size_t stride = (size_t)D.nb + 1;
+    const uint32_t *H = (const uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + (size_t)node * D.nsegs) * stride;
+    // Y_j = lo entries with bucket < j over the node's blocks
+    int Yj = 0;
+    for (int g = lane; g < D.nsegs; g += 64) Yj += (int)(H[(size_t)g * stride + j] & 0xffffu);
+    for (int off = 32; off > 0; off >>= 1) Yj += __shfl_xor(Yj, off);
+    if (Yj == 0) {
+        if (lane == 0) *T = 0;
+        return;
+    }
+    // the first block where Z (entries with z_j before it) + its own count reaches Y_j
+    int Z = 0, seg = D.nsegs - 1;
+    for (int g0 = 0; g0 < D.nsegs; g0 += 64) {
+        const int g = g0 + lane;
+        int c = 0;
+        if (g < D.nsegs) {
+            const uint32_t P = H[(size_t)g * stride + j], tot = H[(size_t)g * stride + D.nb];
+            c = (int)(P & 0xffffu) + (int)(tot >> 16) - (int)(P >> 16);
+        }
+        const int incl = wave_incl_scan(c);
+        const unsigned long long reach = __ballot(g < D.nsegs && Z + incl >= Yj);
+        if (reach) {
+            const int l = __ffsll((long long)reach) - 1;
+            seg = g0 + l;
+            Z += __shfl(incl - c, l);
+            break;
+        }
+        Z += __builtin_amdgcn_readlane(incl, 63);
+    }
+    int need = Yj - Z;
     const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI)) + (size_t)node * D.s + (size_t)seg * D.w;
     const unsigned long long lt = (1ull << lane) - 1;
-    for (int r0 = 0; r0 < D.w; r0 += 64) {
-        const uint32_t e = E[r0 + lane];
-        const int bk = (D.s - 1 - (int)(e & ~kLo31)) >> D.glog;
-        const bool z = (e & kLo31) ? bk < j : bk >= j;
-        const unsigned long long ball = __ballot(z);
-        const int cnt = __popcll(ball);
-        if (need <= cnt) {
-            const bool hit = z && __popcll(ball & lt) == need - 1;
-            const int pos = __ffsll((long long)__ballot(hit)) - 1;
-            if (lane == 0) C.ws[Y.T + (size_t)y * nodes * D.nb + t] = seg * D.w + r0 + pos + 1;
-            return;
+    for (int r0 = 0; r0 < D.w; r0 += 64 * kSweepRounds) {
+        uint32_t ev[kSweepRounds];
+#pragma unroll
+        for (int u = 0; u < kSweepRounds; ++u) ev[u] = E[r0 + 64 * u + lane];
+#pragma unroll
+        for (int u = 0; u < kSweepRounds; ++u) {
+            const uint32_t e = ev[u];
+            const int bk = (D.s - 1 - (int)(e & ~kLo31)) >> D.glog;
+            const bool z = (e & kLo31) ? bk < j : bk >= j;
+            const unsigned long long ball = __ballot(z);
+            const int cnt = __popcll(ball);
+            if (need <= cnt) {
+                const bool hit = z && __popcll(ball & lt) == need - 1;
+                const int pos = __ffsll((long long)__ballot(hit)) - 1;
+                if (lane == 0) *T = seg * D.w + r0 + 64 * u + pos + 1;
+                return;
+            }
+            need -= cnt;
         }
-        need -= cnt;
     }
 }
 
-// One wave per piece of the ant's path: the piece's g x g box (plus the
-// row above and the column to its left) is staged in LDS and lane 0 walks
-// it with the reference's incremental updates, placing the good points.
-// Pieces of a node: q < nb starts at row line j = q (q = 0: the start of
-// the passage at (s-1, 0)), q >= nb at column line i = q - nb + 1.
+// A box entry relative to the box: idx clamped to [base, top] minus base,
+// the lo flag in bit 15.  For r in (base, top], counted31(e, r) of the
+// absolute entry equals counted16 of this one at r - base.
+__device__ __forceinline__ uint16_t box_entry(uint32_t e, int base, int top)
+{
+    const int idx = (int)(e & ~kLo31);
+    return (uint16_t)(((e & kLo31) ? 0x8000u : 0u) | (uint32_t)(min(max(idx, base), top) - base));
+}
+__device__ __forceinline__ int counted16(uint32_t e, int r) { return (int)(((e - (uint32_t)r) >> 15) & 1u); }
+
+// One lane per piece of the ant's path.  Piece q < nb of a node starts at
+// row line j = q (q = 0: the start of the passage at (s-1, 0)), q >= nb at
+// column line i = q - nb + 1; it stays in one g x g box.  Each lane stages
+// its box -- rows r_lo-1 .. r_hi-1 of U and columns c_lo-1 .. c_hi-1 of CI,
+// clamped into the node -- in LDS as 16-bit entries relative to the box, then
+// walks it with the reference's incremental updates, placing the good
+// points.  In box coordinates R = dr - r_lo + 1 (1 .. g while in the box)
+// and X = dc - c_lo + 1 (1 .. nc - 1 before the exit line): U(dr) = bu[R],
+// U(dr-1) = bu[R-1], CI(dc) = bc[X], CI(dc-1) = bc[X-1].
 constexpr int kMaxBoxG = 512;   // g at the largest node (2^22 / kMaxLines)
-__global__ __launch_bounds__(256) void big_walk_kernel(BigBatch B, int d, int flip)
+constexpr size_t walk_lds_bytes(int g) { return (size_t)64 * (2 * g + 3) * sizeof(uint16_t); }
+__global__ __launch_bounds__(64) void big_walk_kernel(BigBatch B, int d, int flip)
 {
     const BigCore &C = B.c[blockIdx.z];
     const BigLayout Y = big_layout(B.L, B.L0);
     const BigDepth D = big_depth(B.L, d);
     const int K = 1 << B.L, s = D.s, nodes = K / s, g = 1 << D.glog, nb = D.nb;
     const int pieces = 2 * nb - 1;
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = blockIdx.x * 64 + threadIdx.x;
     if (t >= nodes * pieces) return;
     const int node = t / pieces, q = t % pieces;
     const size_t v0 = (size_t)node * s;
@@ -818,58 +861,57 @@ __global__ __launch_bounds__(256) void big_walk_kernel(BigBatch B, int d, int fl
     int32_t *Rn = C.ws + (flip ? Y.R : Y.Rn) + v0;
     const int32_t *Trow = C.ws + Y.T + (size_t)node * nb;
     const int32_t *Tcol = C.ws + Y.T + (size_t)nodes * nb + (size_t)node * nb;
-    int dr, dc, delta, r_lo, cb;
+    // T lies in [0, s]; the clamps only keep every access inside the node
+    // after an aborted comb fed undefined input
+    int dr, dc, r_lo, cb;
     if (q < nb) {
         const int Dr = s - q * g;
-        dc = q == 0 ? 0 : Trow[q];
+        dc = q == 0 ? 0 : min(max(Trow[q], 0), s);
         dr = Dr - 1;
-        delta = -(int)counted31(U[dr], dc);
         r_lo = Dr - g;
         cb = dc == 0 ? 0 : (dc - 1) >> D.glog;
     } else {
         const int i = q - nb + 1;
-        const int Tc = Tcol[nb - i];
+        const int Tc = min(max(Tcol[nb - i], 0), s);
         if (Tc == 0) return;   // the ant never reaches this column line
         dr = Tc - 1;
         dc = i * g + 1;
-        delta = -1 + (int)counted31(CI[i * g], dr);
         r_lo = s - (((s - 1 - dr) >> D.glog) + 1) * g;
         cb = i;
     }
-    const int r_hi = r_lo + g;
     const int c_lo = cb ? cb * g + 1 : 0, c_hi = cb == nb - 1 ? s + 1 : (cb + 1) * g + 1;
-    // box: rows r_lo-1 .. r_hi-1, columns c_lo-1 .. c_hi-1 (clamped into the node)
-    __shared__ uint32_t boxU[4][kMaxBoxG + 1], boxC[4][kMaxBoxG + 2];
-    uint32_t *bu = boxU[wave], *bc = boxC[wave];
-    const int nu = g + 1, nc = c_hi - c_lo + 1;
-    for (int i = lane; i < nu; i += 64) bu[i] = U[min(max(r_lo - 1 + i, 0), s - 1)];
-    for (int i = lane; i < nc; i += 64) bc[i] = CI[min(max(c_lo - 1 + i, 0), s - 1)];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane != 0) return;
-    // U(r) = bu[r - r_lo + 1], CI(c) = bc[c - c_lo + 1]; reads clamped like the node-level walk
-    auto Ur = [&](int r) { return bu[min(max(r, r_lo - 1), r_hi - 1) - r_lo + 1]; };
-    auto Cc = [&](int c) { return bc[min(max(c, c_lo - 1), c_hi - 1) - c_lo + 1]; };
-    uint32_t cl = Cc(max(dc - 1, 0)), cc = Cc(min(dc, s - 1));
-    uint32_t ud = Ur(dr), ur = Ur(max(dr - 1, 0));
-    while (dr >= r_lo) {
-        const int d_left = delta - (int)counted31(cl, dr);
-        const int d_down = delta + (int)counted31(ud, dc);
-        if (dc > 0 && d_left < 0 && d_down > 0) Rn[dr] = dc - 1;
-        const bool right = delta < 0 && dc < s;
-        if (right) {
-            delta += (int)counted31(cc, dr);
-            ++dc;
-            if (dc == c_hi) break;   // the next piece starts at this column line
-            cl = cc;
-            cc = Cc(min(dc, s - 1));
-        } else {
-            delta -= dr > 0 ? (int)counted31(ur, dc) : 0;
-            --dr;
-            ud = ur;
-            ur = Ur(max(dr - 1, 0));
-        }
+    const int nc = c_hi - c_lo + 1;   // at most g + 2
+    extern __shared__ uint16_t walk_lds[];
+    uint16_t *bu = walk_lds + (size_t)threadIdx.x * (2 * g + 3), *bc = bu + (g + 1);
+    // U entries hold columns, compared with dc in [c_lo, c_hi - 1]; CI
+    // entries hold rows, compared with dr in [r_lo, r_lo + g - 1]
+#pragma unroll 8
+    for (int i = 0; i <= g; ++i) bu[i] = box_entry(U[min(max(r_lo - 1 + i, 0), s - 1)], c_lo - 1, c_hi);
+#pragma unroll 8
+    for (int i = 0; i < nc; ++i) bc[i] = box_entry(CI[min(max(c_lo - 1 + i, 0), s - 1)], r_lo - 1, r_lo + g);
+    // entering row Dr-1 at T: delta = -[U(Dr-1) counted at T]; stepping right
+    // over column line c = i g at row rho: delta = -1 + [CI(c) counted at rho]
+    int R = dr - r_lo + 1, X = dc - c_lo + 1;
+    const int Xs = s - c_lo + 1;   // X < Xs <=> dc < s
+    int delta = q < nb ? -counted16(bu[R], X) : -1 + counted16(bc[0], R);
+    uint32_t cl = bc[X - 1], cc = bc[X], ud = bu[R], ur = bu[R - 1];
+    // the lanes of a wave take different directions, so each step is written
+    // with selects and reads one LDS entry from a selected index: the new
+    // CI(dc) after a step right, U(dr-1) after a step down
+    while (R >= 1 && X < nc) {
+        const int d_left = delta - counted16(cl, R);
+        const int d_down = delta + counted16(ud, X);
+        if (X + c_lo > 1 && d_left < 0 && d_down > 0) Rn[R + r_lo - 1] = X + c_lo - 2;
+        const bool right = delta < 0 && X < Xs;
+        const int d_right = counted16(cc, R), d_up = R + r_lo > 1 ? counted16(ur, X) : 0;
+        delta = right ? delta + d_right : delta - d_up;
+        X += right ? 1 : 0;
+        R -= right ? 0 : 1;
+        cl = right ? cc : cl;
+        ud = right ? ud : ur;
+        const uint32_t nx = bu[right ? g + 1 + min(X, nc - 1) : max(R - 1, 0)];
+        cc = right ? nx : cc;
+        ur = right ? ur : nx;
     }
 }
 
@@ -1028,7 +1070,8 @@ using namespace lcsb;
 
 // Cores of 2^L points with L >= kBigMinL run over the whole GPU, with
 // leaves of at most 2^kBigLeafMaxL points (at least 2^10: the device-wide
-// partitions work on tiles of 2048 keys inside nodes of 2^(leaf+1)).
+// partitions work on tiles of 2048 keys inside nodes of 2^(leaf+1); at most
+// 2^kLdsCoreL: a leaf is one workgroup's LDS core).
 // Measured on the Zipf pair's five merge levels (k = 2^14, 16 .. 1 products
 // per level): one workgroup per core 15.6 ms for the whole pair; whole-GPU
 // cores with leaves of 2^10 / 2^11 / 2^12 points 12.5 / 12.3 / 12.5 ms.
@@ -1045,8 +1088,8 @@ using namespace lcsb;
 constexpr int kBigMinL = LCS_BIG_MIN_L;
 constexpr int kBigLeafMaxL = LCS_BIG_LEAF_L;
 constexpr int kBigLeafLargeL = LCS_BIG_LEAF_L_LARGE;
-static_assert(kBigMinL >= 11 && kBigLeafMaxL >= 10 && kBigLeafMaxL <= 14 && kBigLeafLargeL >= 10 &&
-                  kBigLeafLargeL <= 14,
+static_assert(kBigMinL >= 11 && kBigMinL <= kLdsCoreL + 1 && kBigLeafMaxL >= 10 && kBigLeafMaxL <= kLdsCoreL &&
+                  kBigLeafLargeL >= 10 && kBigLeafLargeL <= kLdsCoreL,
               "big-core leaf sizes");
 static int big_leaf_L(int L)
 {
@@ -1058,8 +1101,7 @@ static bool is_big(int L) { return L >= kBigMinL; }
 size_t lcsb_core_scratch_words(int32_t k)
 {
     const int L = lcsb_core_L(k);
-    const size_t K = (size_t)1 << L;
-    if (!is_big(L)) return (size_t)2 * L * K;   // parents of P and Q per depth
+    if (!is_big(L)) return 0;   // the one-workgroup core keeps everything in LDS
     return big_layout(L, big_leaf_L(L)).total;
 }
 
@@ -1083,7 +1125,7 @@ static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
         hipLaunchKernelGGL(big_split_scan_kernel, dim3(1u << d, 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
         hipLaunchKernelGGL(big_split_scatter_kernel, tiles, dim3(256), 0, stream, B, d);
     }
-    hipLaunchKernelGGL(big_leaf_kernel, dim3(1u << d0, 1, (unsigned)cnt), dim3(kThreads), lds_core_bytes(1 << L0),
+    hipLaunchKernelGGL(big_leaf_kernel, dim3(1u << d0, 1, (unsigned)cnt), dim3(kThreads), lds_core_bytes(L0),
                        stream, B);
     int flip = 0;
     for (int d = d0 - 1; d >= 0; --d) {
@@ -1091,12 +1133,10 @@ static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
         const int lines = (K / D.s) * D.nb, pieces = (K / D.s) * (2 * D.nb - 1);
         hipLaunchKernelGGL(big_lift_kernel, lin, dim3(256), 0, stream, B, d, flip);
         hipLaunchKernelGGL(big_hist_kernel, dim3((unsigned)(K / D.w), 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
-        hipLaunchKernelGGL(big_bound_kernel, dim3((unsigned)((lines + 255) / 256), 2, (unsigned)cnt), dim3(256), 0,
-                           stream, B, d);
         hipLaunchKernelGGL(big_sweep_kernel, dim3((unsigned)((lines + 3) / 4), 2, (unsigned)cnt), dim3(256), 0, stream,
                            B, d);
-        hipLaunchKernelGGL(big_walk_kernel, dim3((unsigned)((pieces + 3) / 4), 1, (unsigned)cnt), dim3(256), 0, stream,
-                           B, d, flip);
+        hipLaunchKernelGGL(big_walk_kernel, dim3((unsigned)((pieces + 63) / 64), 1, (unsigned)cnt), dim3(64),
+                           walk_lds_bytes(1 << D.glog), stream, B, d, flip);
         flip ^= 1;
     }
     hipLaunchKernelGGL(big_out_kernel, lin, dim3(256), 0, stream, B, flip);
@@ -1111,9 +1151,11 @@ int lcsb_init(void)
     static int rc = 0;
     std::call_once(once, [] {
         if (hipFuncSetAttribute((const void *)braid_core_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_core_bytes(kLdsCoreK)) != hipSuccess ||
+                                (int)lds_core_bytes(kLdsCoreL)) != hipSuccess ||
             hipFuncSetAttribute((const void *)big_leaf_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_core_bytes(kLdsCoreK)) != hipSuccess)
+                                (int)lds_core_bytes(kLdsCoreL)) != hipSuccess ||
+            hipFuncSetAttribute((const void *)big_walk_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)walk_lds_bytes(kMaxBoxG)) != hipSuccess)
             rc = -5;
     });
     return rc;
@@ -1132,12 +1174,13 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
     for (size_t base = 0; base < small.size(); base += kCoreBatch) {
         CoreBatch batch{};
         const int cnt = (int)std::min<size_t>(small.size() - base, kCoreBatch);
-        int max_K = 1;
+        int max_L = 0;
         for (int i = 0; i < cnt; ++i) {
             batch.d[i] = small[base + (size_t)i];
-            if ((1 << batch.d[i].L) > max_K) max_K = 1 << batch.d[i].L;
+            max_L = std::max(max_L, (int)batch.d[i].L);
         }
-        hipLaunchKernelGGL(braid_core_lds_kernel, dim3(cnt), dim3(kThreads), lds_core_bytes(max_K), stream, batch);
+        if (max_L > kLdsCoreL) return -34;
+        hipLaunchKernelGGL(braid_core_lds_kernel, dim3(cnt), dim3(kThreads), lds_core_bytes(max_L), stream, batch);
     }
     // big cores: products of one padded size share each launch of the sequence
     std::sort(big.begin(), big.end(), [](const BraidCoreDesc &x, const BraidCoreDesc &y) { return x.L < y.L; });

@@ -314,21 +314,24 @@ int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, h
 
 // Strips a single pair is cut into.  A thin pair (m <= kDefaultStripMaxM)
 // whose bands alone would leave SIMDs idle is cut until the strips' bands
-// fill them.  A tall pair whose bands already fill every SIMD is cut in two
-// when it is wide: the persistent waves then comb each strip's lower bands
-// as a second round, so the chain's fill (one band-to-band lag per band
-// before the last band starts) is paid for half the bands, at the price of
-// one whole-GPU steady-ant merge with k = m.  At m = n = 2^20 two strips
-// took 141.3 ms against 146.3 ms for one (the same kernels), four 143.6 ms
-// and eight 154.4 ms (more merge levels).
+// fill them.  A tall pair whose bands already fill every SIMD is cut when it
+// is wide: the persistent waves then comb each strip's lower bands as later
+// rounds, so the chain's fill (one band-to-band lag per band before the
+// last band starts) is paid for a fraction of the bands, at the price of
+// whole-GPU steady-ant merges with k = m (S - 1 of them).  At m = n = 2^20
+// (the same kernels, same box): one strip 145.0 ms, two 138.0 (chain 136.0
+// + merge 2.0), four 136.7 (131.6 + 5.1), eight 140.3 (129.3 + 11.0).
 constexpr int32_t kDefaultStripMaxM = 32768;
-constexpr int32_t kTallStripMinN = 1 << 18;
+constexpr int32_t kTallStripMinN = 1 << 18;    // two strips from here
+constexpr int32_t kTallStrip4MinN = 1 << 20;   // four strips from here
 int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 {
     const long long simds = 4LL * ctx->num_cus;
     const long long bands = bands_of(m, n, tm);
-    if (m > kDefaultStripMaxM)
-        return (m <= kMaxCoreK && n >= kTallStripMinN && bands >= simds) ? 2 : 1;
+    if (m > kDefaultStripMaxM) {
+        if (m > kMaxCoreK || bands < simds || n < kTallStripMinN) return 1;
+        return n >= kTallStrip4MinN ? 4 : 2;
+    }
     if (n < 2 * 8192) return 1;
     int s = 1;
     while ((long long)s * bands < simds && s < 64 && (long long)n / (2 * s) >= 8192) s *= 2;

@@ -22,9 +22,11 @@ def test_core_random_permutations(ctx):
             assert np.array_equal(got, exp), (k, glue)
 
 
-# 16384 is the largest core kept wholly in one workgroup's LDS; larger
-# cores run over the whole GPU (top depths device-wide, leaves of 2^14)
-@pytest.mark.parametrize("k", [1, 2, 31, 32, 33, 100, 1000, 4097, 16384, 16385, 20000, 32768])
+# 2048 is the largest core kept wholly in one workgroup's LDS (keys,
+# products and every depth's parents); larger cores run over the whole GPU
+# (top depths device-wide, leaves of 2^11 points)
+@pytest.mark.parametrize("k", [1, 2, 31, 32, 33, 100, 1000, 2047, 2048, 2049, 4096, 4097, 16384, 16385, 20000,
+                               32768])
 def test_core_sizes(ctx, k):
     rng = np.random.default_rng(k)
     p = rng.permutation(k).astype(np.int32)
