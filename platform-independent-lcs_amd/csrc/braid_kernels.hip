@@ -55,10 +55,11 @@ constexpr int kLdsCoreK = 1 << kLdsCoreL;
 // keeps 2 x kMaxChunks bucket counters in LDS for the boundary sweeps
 constexpr int kMaxChunks = 128;
 // dynamic LDS of a core of 2^L points: keys / U, CI, R, Rn (4K uint16), the
-// K/64+1 group counts, then the parents of P and of Q at each depth (2 L K uint16)
+// K/64+1 group counts of P and of Q, then the parents of P and of Q at each
+// depth (2 L K uint16)
 constexpr size_t lds_core_bytes(int L)
 {
-    return (size_t)4 * (1 << L) * sizeof(uint16_t) + ((size_t)(1 << L) / 64 + 1) * 4 +
+    return (size_t)4 * (1 << L) * sizeof(uint16_t) + 2 * ((size_t)(1 << L) / 64 + 1) * 4 +
            (size_t)2 * L * (1 << L) * sizeof(uint16_t);
 }
 
@@ -305,54 +306,54 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     __syncthreads();
 }
 
-// One depth of the top-down split with the keys in LDS: a stable partition
-// of every aligned segment of size s by bit `bit` (zeros first), the ranks
-// from wave ballots and one scan of the 64-key group counts.
-// par[pos] = old position (LDS).  K >= 64.  A wave takes the groups
-// wave, wave + 16, ...; U of them at a time, with every load of the U groups
-// issued before the first ballot, so their LDS latencies overlap (the
-// compiler does not unroll a loop around a ballot by itself).
+// One depth of the top-down split with the keys in LDS, for P (waves
+// 0 .. kSideW-1) and Q (the other half) at once: a stable partition of every
+// aligned segment of size s by bit `bit` (zeros first), the ranks from wave
+// ballots; par[pos] = old position.  Each wave counts the ones of its 64-key
+// groups (gc, one word per group); after one barrier every wave scans all
+// of its side's group counts itself (K <= 4096: at most 64 groups, one DPP
+// scan), so a depth costs two barriers.  A wave takes the groups w, w +
+// kSideW, ..., U at a time, with every load of the U groups issued before
+// the first ballot, so their LDS latencies overlap (the compiler does not
+// unroll a loop around a ballot by itself).  K >= 64.
+constexpr int kSideW = kThreads / 128;
+static_assert(kLdsCoreK <= 64 * 64, "one DPP scan covers a side's group counts");
 template <int U>
-__device__ __forceinline__ void split_count_groups(const uint16_t *key, int bit, int32_t *gpre, int g0, int lane)
+__device__ __forceinline__ void split_count_groups(const uint16_t *key, int bit, int32_t *gc, int g0, int lane)
 {
     uint16_t kv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) kv[u] = key[(g0 + u * (kThreads / 64)) * 64 + lane];
+    for (int u = 0; u < U; ++u) kv[u] = key[(g0 + u * kSideW) * 64 + lane];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const unsigned long long m = __ballot((kv[u] >> bit) & 1);
-        if (lane == 0) gpre[g0 + u * (kThreads / 64)] = __popcll(m);
+        if (lane == 0) gc[g0 + u * kSideW] = __popcll(m);
     }
 }
 
+// excl = exclusive prefix of the group counts at lane g, total = all ones
 template <int U>
 __device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t *key_new, uint16_t *par, int s,
-                                                   int bit, const int32_t *gpre, int g0, int lane)
+                                                   int bit, int excl, int total, int ngroups, int g0, int lane)
 {
     const unsigned long long lt = (1ull << lane) - 1;
     uint16_t kv[U];
-    int gp[U], gs0[U], gs1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kv[u] = key[(g0 + u * kSideW) * 64 + lane];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-        const int g = g0 + u * (kThreads / 64), x = g * 64 + lane;
-        kv[u] = key[x];
-        if (s >= 64) {
-            const int gs = (x & ~(s - 1)) >> 6;
-            gp[u] = gpre[g];
-            gs0[u] = gpre[gs];
-            gs1[u] = gpre[gs + (s >> 6)];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int g = g0 + u * (kThreads / 64), x = g * 64 + lane;
+        const int g = g0 + u * kSideW, x = g * 64 + lane;
         const int b = (kv[u] >> bit) & 1;
         const unsigned long long m = __ballot(b);
         const int seg = x & ~(s - 1);
         int before, ones;
         if (s >= 64) {
-            before = gp[u] - gs0[u] + __popcll(m & lt);
-            ones = gs1[u] - gs0[u];
+            // the segment's groups are gs .. ge-1; prefixes read from the scan
+            const int gs = seg >> 6, ge = gs + (s >> 6);
+            const int p_g = __builtin_amdgcn_readlane(excl, g), p_s = __builtin_amdgcn_readlane(excl, gs);
+            const int p_e = ge < ngroups ? __builtin_amdgcn_readlane(excl, ge) : total;
+            before = p_g - p_s + __popcll(m & lt);
+            ones = p_e - p_s;
         } else {
             const unsigned long long segmask = ((1ull << s) - 1) << (lane & ~(s - 1));
             before = __popcll(m & lt & segmask);
@@ -364,30 +365,28 @@ __device__ __forceinline__ void split_place_groups(const uint16_t *key, uint16_t
     }
 }
 
-__device__ void split_depth_lds(const uint16_t *key, uint16_t *key_new, uint16_t *par, int K, int s, int bit,
-                                int32_t *gpre)
+__device__ __forceinline__ void split_depth_pq(const uint16_t *keyP, uint16_t *keyP_new, uint16_t *parP,
+                                               const uint16_t *keyQ, uint16_t *keyQ_new, uint16_t *parQ,
+                                               int32_t *gcP, int32_t *gcQ, int K, int s, int bit)
 {
-    constexpr int kW = kThreads / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool q_side = wave >= kSideW;
+    const int w = q_side ? wave - kSideW : wave;
+    const uint16_t *key = q_side ? keyQ : keyP;
+    uint16_t *key_new = q_side ? keyQ_new : keyP_new, *par = q_side ? parQ : parP;
+    int32_t *gc = q_side ? gcQ : gcP;
     const int ngroups = K >> 6;
-    int g = wave;
-    for (; g + 3 * kW < ngroups; g += 4 * kW) split_count_groups<4>(key, bit, gpre, g, lane);
-    for (; g < ngroups; g += kW) split_count_groups<1>(key, bit, gpre, g, lane);
+    int g = w;
+    for (; g + 3 * kSideW < ngroups; g += 4 * kSideW) split_count_groups<4>(key, bit, gc, g, lane);
+    for (; g < ngroups; g += kSideW) split_count_groups<1>(key, bit, gc, g, lane);
     __syncthreads();
-    if (wave == 0) {  // exclusive scan of the group counts, gpre[ngroups] = total
-        int carry = 0;
-        for (int base = 0; base < ngroups; base += 64) {
-            const int c = base + lane < ngroups ? gpre[base + lane] : 0;
-            const int incl = wave_incl_scan(c);
-            if (base + lane < ngroups) gpre[base + lane] = carry + incl - c;
-            carry += __builtin_amdgcn_readlane(incl, 63);
-        }
-        if (lane == 0) gpre[ngroups] = carry;
-    }
-    __syncthreads();
-    g = wave;
-    for (; g + 3 * kW < ngroups; g += 4 * kW) split_place_groups<4>(key, key_new, par, s, bit, gpre, g, lane);
-    for (; g < ngroups; g += kW) split_place_groups<1>(key, key_new, par, s, bit, gpre, g, lane);
+    const int c = lane < ngroups ? gc[lane] : 0;
+    const int incl = wave_incl_scan(c);
+    const int excl = incl - c, total = __builtin_amdgcn_readlane(incl, 63);
+    g = w;
+    for (; g + 3 * kSideW < ngroups; g += 4 * kSideW)
+        split_place_groups<4>(key, key_new, par, s, bit, excl, total, ngroups, g, lane);
+    for (; g < ngroups; g += kSideW) split_place_groups<1>(key, key_new, par, s, bit, excl, total, ngroups, g, lane);
     __syncthreads();
 }
 
@@ -402,35 +401,30 @@ __device__ __forceinline__ void core_lds_body(const int32_t *P, const int32_t *Q
                                               uint16_t *lds, int32_t *bT, int32_t *bD, int *hist)
 {
     const int K = 1 << L, mask = K - 1;
-    uint16_t *key = lds, *key2 = lds + K;             // top-down (P, then Q)
-    uint16_t *U = lds, *CI = lds + K;                 // bottom-up (reuses the key space)
+    uint16_t *keyP = lds, *keyP2 = lds + K;                       // top-down
+    uint16_t *keyQ = lds + 2 * (size_t)K, *keyQ2 = lds + 3 * (size_t)K;
+    uint16_t *U = lds, *CI = lds + K;                             // bottom-up (reuses the key space)
     uint16_t *R = lds + 2 * (size_t)K, *Rn = lds + 3 * (size_t)K;
-    int32_t *gpre = (int32_t *)(lds + 4 * (size_t)K);
-    uint16_t *parP = lds + 4 * (size_t)K + 2 * ((size_t)K / 64 + 1);   // L * K
+    int32_t *gcP = (int32_t *)(lds + 4 * (size_t)K), *gcQ = gcP + (K / 64 + 1);
+    uint16_t *parP = lds + 4 * (size_t)K + 4 * ((size_t)K / 64 + 1);   // L * K
     uint16_t *parQ = parP + (size_t)L * K;                              // L * K
 
-    // P ordered by row (key = shared column)
-    for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(x < k ? (P[x] & mask) : x);
-    __syncthreads();
-    for (int d = 0; d < L; ++d) {
-        split_depth_lds(key, key2, parP + (size_t)d * K, K, K >> d, L - d - 1, gpre);
-        uint16_t *t = key; key = key2; key2 = t;
-    }
-    // Q ordered by column (key = shared row)
-    key = lds;
-    key2 = lds + K;
+    // P ordered by row (key = shared column), Q ordered by column (key = shared row)
+    for (int x = threadIdx.x; x < K; x += kThreads) keyP[x] = (uint16_t)(x < k ? (P[x] & mask) : x);
     if (QKEY) {
-        for (int x = threadIdx.x; x < K; x += kThreads) key[x] = (uint16_t)(Q[x] & mask);
+        for (int x = threadIdx.x; x < K; x += kThreads) keyQ[x] = (uint16_t)(Q[x] & mask);
     } else {
         for (int x = threadIdx.x; x < K; x += kThreads)
-            if (x >= k) key[x] = (uint16_t)x;
+            if (x >= k) keyQ[x] = (uint16_t)x;
         for (int r = threadIdx.x; r < k; r += kThreads)
-            if ((uint32_t)Q[r] < (uint32_t)k) key[Q[r]] = (uint16_t)r;
+            if ((uint32_t)Q[r] < (uint32_t)k) keyQ[Q[r]] = (uint16_t)r;
     }
     __syncthreads();
     for (int d = 0; d < L; ++d) {
-        split_depth_lds(key, key2, parQ + (size_t)d * K, K, K >> d, L - d - 1, gpre);
-        uint16_t *t = key; key = key2; key2 = t;
+        split_depth_pq(keyP, keyP2, parP + (size_t)d * K, keyQ, keyQ2, parQ + (size_t)d * K, gcP, gcQ, K, K >> d,
+                       L - d - 1);
+        uint16_t *t = keyP; keyP = keyP2; keyP2 = t;
+        t = keyQ; keyQ = keyQ2; keyQ2 = t;
     }
     for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
     __syncthreads();
@@ -506,18 +500,29 @@ struct BigDepth {
     int32_t s, nb, glog, w, nsegs;   // w entries per histogram block, nsegs per node
 };
 
+// Histogram blocks of w entries: every open boundary sweeps (on average)
+// half a block, nb w / 2 entries per node and direction, while the
+// blocks' prefixes cost about 2 (s / w)(nb + 1) words written and read;
+// w = 2 sqrt(s) balances the two (clamped to [512, 8192], at most s = 2^e).
+__host__ __device__ constexpr int big_w_log(int e)
+{
+    const int wl = 1 + (e + 1) / 2;
+    return wl < 9 ? 9 : (wl > 13 ? 13 : wl);
+}
+constexpr bool big_blocks_fit(int max_blocks)
+{
+    for (int e = 11; e <= kMaxCoreL; ++e)
+        if ((1 << (e - (big_w_log(e) < e ? big_w_log(e) : e))) > max_blocks) return false;
+    return true;
+}
+
 __host__ __device__ inline BigDepth big_depth(int L, int d)
 {
     BigDepth D;
     D.s = 1 << (L - d);
     D.nb = D.s / kBoxG < kMaxLines ? D.s / kBoxG : kMaxLines;
     D.glog = __builtin_ctz((unsigned)(D.s / D.nb));
-    // Histogram blocks of w entries: every open boundary sweeps (on average)
-    // half a block, nb w / 2 entries per node and direction, while the
-    // blocks' prefixes cost about 2 (s / w)(nb + 1) words written and read;
-    // w = 2 sqrt(s) balances the two (clamped to [512, 8192], at most s).
-    const int e = L - d, wl = 1 + (e + 1) / 2;
-    D.w = 1 << (wl < 9 ? 9 : (wl > 13 ? 13 : wl));
+    D.w = 1 << big_w_log(L - d);
     if (D.w > D.s) D.w = D.s;
     D.nsegs = D.s / D.w;
     return D;
@@ -717,7 +722,7 @@ __global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
     const int K = 1 << B.L, y = blockIdx.y;
     const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI));
     __shared__ uint32_t h[kMaxLines];
-    __shared__ uint32_t sums[256];
+    __shared__ uint32_t sums[4];
     for (int j = threadIdx.x; j < D.nb; j += 256) h[j] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * D.w;
@@ -727,43 +732,136 @@ __global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
         atomicAdd(&h[bk], (e & kLo31) ? 1u : 0x10000u);
     }
     __syncthreads();
-    const int per = D.nb / 256 > 0 ? D.nb / 256 : 1;   // nb is a power of two
-    const int lo = min((int)threadIdx.x * per, D.nb), hi = min(lo + per, D.nb);
-    uint32_t run = 0;
-    for (int j = lo; j < hi; ++j) run += h[j];
-    sums[threadIdx.x] = run;
+    // exclusive prefix in place: each of nw waves scans its nb / nw
+    // consecutive buckets 64 at a time (DPP scan plus a carry; the packed
+    // halves stay below 2^16, so they add without carrying into each
+    // other), then the prefix leaves with consecutive threads on
+    // consecutive buckets plus the totals of the waves before
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = D.nb >= 256 ? 4 : 1, chunk = D.nb / nw;   // nb is a power of two
+    if (wave < nw) {
+        uint32_t carry = 0;
+        const int jb = wave * chunk, je = jb + chunk;
+        for (int j0 = jb; j0 < je; j0 += 64) {
+            const int j = j0 + lane;
+            const uint32_t c = j < je ? h[j] : 0u;
+            const uint32_t incl = (uint32_t)wave_incl_scan((int)c);
+            if (j < je) h[j] = carry + incl - c;
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        }
+        if (lane == 0) sums[wave] = carry;
+    }
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
+    // per (y, node): the nsegs blocks' prefix rows of nb words (each block
+    // writes its own row, coalesced), then the nsegs block totals in a row
+    // of their own (read coalesced by the search)
+    const int node = blockIdx.x / D.nsegs, g = blockIdx.x % D.nsegs, nodes = K / D.s;
+    uint32_t *rows = (uint32_t *)(C.ws + Y.hist) + ((size_t)y * nodes + node) * D.nsegs * (size_t)(D.nb + 1);
+    uint32_t *out = rows + (size_t)g * D.nb;
+    const uint32_t w1 = sums[0], w2 = w1 + (nw > 1 ? sums[1] : 0u), w3 = w2 + (nw > 2 ? sums[2] : 0u);
+    for (int j = threadIdx.x; j < D.nb; j += 256) {
+        const int wj = j / chunk;
+        out[j] = h[j] + (wj == 0 ? 0u : wj == 1 ? w1 : wj == 2 ? w2 : w3);
     }
-    uint32_t acc = sums[threadIdx.x] - run;
-    uint32_t *out = (uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + blockIdx.x) * (size_t)(D.nb + 1);
-    for (int j = lo; j < hi; ++j) {
-        out[j] = acc;
-        acc += h[j];
-    }
-    if (threadIdx.x == 255) out[D.nb] = sums[255];
+    if (threadIdx.x == 0) rows[(size_t)D.nsegs * D.nb + g] = nw > 1 ? w3 + sums[3] : w1;
 }
 
-// One wave per (y, node, boundary j): Y_j from the blocks' lo prefixes,
-// the block where the running count reaches Y_j (a wave scan over the
-// blocks, 64 at a time), then a sweep of that block for the entry where the
-// count reaches the rank still needed; T = that entry's position + 1.
-// Boundaries with j = 0 or Y_j = 0 get T = 0.  The block (w >= 512 entries,
-// a power of two) is read kSweepRounds x 64 entries at a time, all loads
-// issued before the first ballot, so one memory latency covers that many
-// rounds of the scan.
-constexpr int kSweepRounds = 8;
-__global__ __launch_bounds__(256) void big_sweep_kernel(BigBatch B, int d)
+// The block of a node where the running count of z_j entries reaches Y_j.
+// Hj[g * nb] = block g's prefix at j (lo | hi << 16 entries with bucket <
+// j), Ht[g] = its totals: Y_j sums the lo parts; the block holds lo(P) +
+// hi(total) - hi(P) z_j entries.  Lane l holds blocks l, l + 64, ... (R
+// rounds, 64 R >= nsegs), all loaded in one batch from clamped (valid)
+// blocks and masked afterwards; the rounds' wave scans find the first block
+// where Z (z_j entries before it) plus its own count reaches Y_j.
+template <int R>
+__device__ __forceinline__ void block_search(const uint32_t *Hj, const uint32_t *Ht, int nb, int nsegs, int lane,
+                                             int &Yj, int &Z, int &seg)
+{
+    uint32_t Pv[R], Tv[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int g = min(64 * u + lane, nsegs - 1);
+        Pv[u] = Hj[(size_t)g * nb];
+        Tv[u] = Ht[g];
+    }
+    int lo_sum = 0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const bool mine = 64 * u + lane < nsegs;
+        Pv[u] = mine ? Pv[u] : 0u;
+        Tv[u] = mine ? Tv[u] : 0u;
+        lo_sum += (int)(Pv[u] & 0xffffu);
+    }
+    Yj = lo_sum;
+    for (int off = 32; off > 0; off >>= 1) Yj += __shfl_xor(Yj, off);
+    Z = 0;
+    seg = nsegs - 1;
+    bool found = false;   // uniform
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int c = (int)(Pv[u] & 0xffffu) + (int)(Tv[u] >> 16) - (int)(Pv[u] >> 16);
+        const int incl = wave_incl_scan(c);
+        const unsigned long long reach = found ? 0ull : __ballot(Z + incl >= Yj);
+        if (reach) {
+            const int l = __ffsll((long long)reach) - 1;
+            seg = min(64 * u + l, nsegs - 1);
+            Z += __shfl(incl - c, l);
+            found = true;
+        } else if (!found) {
+            Z += __builtin_amdgcn_readlane(incl, 63);
+        }
+    }
+}
+
+// The entry of a block of w entries (w >= 64 RS, a multiple of it) where
+// the count of z_j entries reaches `need`: the block is read RS x 64
+// entries at a time, all loads issued before the first ballot, so one
+// memory latency covers that much of it.  Returns the entry's position
+// + 1 in the block (0 if the count never gets there).
+template <int RS>
+__device__ __forceinline__ int sweep_block(const uint32_t *E, int w, int s, int glog, int j, int need, int lane)
+{
+    const unsigned long long lt = (1ull << lane) - 1;
+    for (int r0 = 0; r0 < w; r0 += 64 * RS) {
+        uint32_t ev[RS];
+#pragma unroll
+        for (int u = 0; u < RS; ++u) ev[u] = E[r0 + 64 * u + lane];
+#pragma unroll
+        for (int u = 0; u < RS; ++u) {
+            const uint32_t e = ev[u];
+            const int bk = (s - 1 - (int)(e & ~kLo31)) >> glog;
+            const bool z = (e & kLo31) ? bk < j : bk >= j;
+            const unsigned long long ball = __ballot(z);
+            const int cnt = __popcll(ball);
+            if (need <= cnt) {
+                const bool hit = z && __popcll(ball & lt) == need - 1;
+                return r0 + 64 * u + __ffsll((long long)__ballot(hit));
+            }
+            need -= cnt;
+        }
+    }
+    return 0;
+}
+
+// One wave per (y, node, boundary j): Y_j and the block where the running
+// count reaches it (block_search, at most kSegRounds x 64 blocks per node),
+// then a sweep of that block (sweep_block) for the entry where the count
+// reaches the rank still needed; T = that entry's position + 1.
+// Boundaries with j = 0 or Y_j = 0 get T = 0.
+constexpr int kSegRounds = 16;
+static_assert(big_blocks_fit(64 * kSegRounds), "histogram blocks per node");
+#ifndef LCS_SWEEP_WAVES
+#define LCS_SWEEP_WAVES 4
+#endif
+constexpr int kSweepWaves = LCS_SWEEP_WAVES;   // waves (boundaries) per workgroup
+__global__ __launch_bounds__(64 * kSweepWaves) void big_sweep_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
     const BigLayout Y = big_layout(B.L, B.L0);
     const BigDepth D = big_depth(B.L, d);
     const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // the wave's boundary, made scalar so that its addresses stay in SGPRs
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kSweepWaves + (threadIdx.x >> 6)), lane = threadIdx.x & 63;
     if (t >= nodes * D.nb) return;
     const int node = t / D.nb, j = t % D.nb;
     int32_t *T = C.ws + Y.T + (size_t)y * nodes * D.nb + t;
@@ -771,58 +869,23 @@ __global__ __launch_bounds__(256) void big_sweep_kernel(BigBatch B, int d)
         if (lane == 0) *T = 0;
         return;
     }
-    const size_t stride = (size_t)D.nb + 1;
-    const uint32_t *H = (const uint32_t *)(C.ws + Y.hist) + ((size_t)y * (K / D.w) + (size_t)node * D.nsegs) * stride;
-    // Y_j = lo entries with bucket < j over the node's blocks
-    int Yj = 0;
-    for (int g = lane; g < D.nsegs; g += 64) Yj += (int)(H[(size_t)g * stride + j] & 0xffffu);
-    for (int off = 32; off > 0; off >>= 1) Yj += __shfl_xor(Yj, off);
+    const uint32_t *H = (const uint32_t *)(C.ws + Y.hist) + ((size_t)y * nodes + node) * D.nsegs * (size_t)(D.nb + 1);
+    const uint32_t *Hj = H + j, *Ht = H + (size_t)D.nsegs * D.nb;
+    int Yj, Z, seg;
+    if (D.nsegs <= 64) block_search<1>(Hj, Ht, D.nb, D.nsegs, lane, Yj, Z, seg);
+    else if (D.nsegs <= 128) block_search<2>(Hj, Ht, D.nb, D.nsegs, lane, Yj, Z, seg);
+    else if (D.nsegs <= 256) block_search<4>(Hj, Ht, D.nb, D.nsegs, lane, Yj, Z, seg);
+    else if (D.nsegs <= 512) block_search<8>(Hj, Ht, D.nb, D.nsegs, lane, Yj, Z, seg);
+    else block_search<kSegRounds>(Hj, Ht, D.nb, D.nsegs, lane, Yj, Z, seg);
     if (Yj == 0) {
         if (lane == 0) *T = 0;
         return;
     }
-    // the first block where Z (entries with z_j before it) + its own count reaches Y_j
-    int Z = 0, seg = D.nsegs - 1;
-    for (int g0 = 0; g0 < D.nsegs; g0 += 64) {
-        const int g = g0 + lane;
-        int c = 0;
-        if (g < D.nsegs) {
-            const uint32_t P = H[(size_t)g * stride + j], tot = H[(size_t)g * stride + D.nb];
-            c = (int)(P & 0xffffu) + (int)(tot >> 16) - (int)(P >> 16);
-        }
-        const int incl = wave_incl_scan(c);
-        const unsigned long long reach = __ballot(g < D.nsegs && Z + incl >= Yj);
-        if (reach) {
-            const int l = __ffsll((long long)reach) - 1;
-            seg = g0 + l;
-            Z += __shfl(incl - c, l);
-            break;
-        }
-        Z += __builtin_amdgcn_readlane(incl, 63);
-    }
-    int need = Yj - Z;
     const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI)) + (size_t)node * D.s + (size_t)seg * D.w;
-    const unsigned long long lt = (1ull << lane) - 1;
-    for (int r0 = 0; r0 < D.w; r0 += 64 * kSweepRounds) {
-        uint32_t ev[kSweepRounds];
-#pragma unroll
-        for (int u = 0; u < kSweepRounds; ++u) ev[u] = E[r0 + 64 * u + lane];
-#pragma unroll
-        for (int u = 0; u < kSweepRounds; ++u) {
-            const uint32_t e = ev[u];
-            const int bk = (D.s - 1 - (int)(e & ~kLo31)) >> D.glog;
-            const bool z = (e & kLo31) ? bk < j : bk >= j;
-            const unsigned long long ball = __ballot(z);
-            const int cnt = __popcll(ball);
-            if (need <= cnt) {
-                const bool hit = z && __popcll(ball & lt) == need - 1;
-                const int pos = __ffsll((long long)__ballot(hit)) - 1;
-                if (lane == 0) *T = seg * D.w + r0 + 64 * u + pos + 1;
-                return;
-            }
-            need -= cnt;
-        }
-    }
+    // w is 512 or a multiple of 1024
+    const int pos = D.w == 512 ? sweep_block<8>(E, D.w, D.s, D.glog, j, Yj - Z, lane)
+                               : sweep_block<16>(E, D.w, D.s, D.glog, j, Yj - Z, lane);
+    if (lane == 0 && pos) *T = seg * D.w + pos;
 }
 
 // A box entry relative to the box: idx clamped to [base, top] minus base,
@@ -885,9 +948,9 @@ __global__ __launch_bounds__(64) void big_walk_kernel(BigBatch B, int d, int fli
     uint16_t *bu = walk_lds + (size_t)threadIdx.x * (2 * g + 3), *bc = bu + (g + 1);
     // U entries hold columns, compared with dc in [c_lo, c_hi - 1]; CI
     // entries hold rows, compared with dr in [r_lo, r_lo + g - 1]
-#pragma unroll 8
+#pragma unroll 32
     for (int i = 0; i <= g; ++i) bu[i] = box_entry(U[min(max(r_lo - 1 + i, 0), s - 1)], c_lo - 1, c_hi);
-#pragma unroll 8
+#pragma unroll 32
     for (int i = 0; i < nc; ++i) bc[i] = box_entry(CI[min(max(c_lo - 1 + i, 0), s - 1)], r_lo - 1, r_lo + g);
     // entering row Dr-1 at T: delta = -[U(Dr-1) counted at T]; stepping right
     // over column line c = i g at row rho: delta = -1 + [CI(c) counted at rho]
@@ -1133,7 +1196,8 @@ static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
         const int lines = (K / D.s) * D.nb, pieces = (K / D.s) * (2 * D.nb - 1);
         hipLaunchKernelGGL(big_lift_kernel, lin, dim3(256), 0, stream, B, d, flip);
         hipLaunchKernelGGL(big_hist_kernel, dim3((unsigned)(K / D.w), 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
-        hipLaunchKernelGGL(big_sweep_kernel, dim3((unsigned)((lines + 3) / 4), 2, (unsigned)cnt), dim3(256), 0, stream,
+        hipLaunchKernelGGL(big_sweep_kernel, dim3((unsigned)((lines + kSweepWaves - 1) / kSweepWaves), 2, (unsigned)cnt),
+                           dim3(64 * kSweepWaves), 0, stream,
                            B, d);
         hipLaunchKernelGGL(big_walk_kernel, dim3((unsigned)((pieces + 63) / 64), 1, (unsigned)cnt), dim3(64),
                            walk_lds_bytes(1 << D.glog), stream, B, d, flip);
