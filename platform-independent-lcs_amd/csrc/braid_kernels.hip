@@ -490,11 +490,6 @@ struct BigCore {
     int32_t *ws;            // lcsb_big_ws_words(k, L0) words
     int32_t k;
 };
-struct BigBatch {
-    BigCore c[kBigBatch];
-    int32_t L, L0;
-};
-
 // grid of one top depth (node size s): lines every g = s / nb
 struct BigDepth {
     int32_t s, nb, glog, w, nsegs;   // w entries per histogram block, nsegs per node
@@ -503,11 +498,16 @@ struct BigDepth {
 // Histogram blocks of w entries: every open boundary sweeps (on average)
 // half a block, nb w / 2 entries per node and direction, while the
 // blocks' prefixes cost about 2 (s / w)(nb + 1) words written and read;
-// w = 2 sqrt(s) balances the two (clamped to [512, 8192], at most s = 2^e).
+// w = 2 sqrt(s) balances the two (clamped to [2^LCS_BIG_MIN_W_LOG, 8192],
+// at most s = 2^e).
+#ifndef LCS_BIG_MIN_W_LOG
+#define LCS_BIG_MIN_W_LOG 9
+#endif
+static_assert(LCS_BIG_MIN_W_LOG >= 7 && LCS_BIG_MIN_W_LOG <= 11, "histogram block width");
 __host__ __device__ constexpr int big_w_log(int e)
 {
     const int wl = 1 + (e + 1) / 2;
-    return wl < 9 ? 9 : (wl > 13 ? 13 : wl);
+    return wl < LCS_BIG_MIN_W_LOG ? LCS_BIG_MIN_W_LOG : (wl > 13 ? 13 : wl);
 }
 constexpr bool big_blocks_fit(int max_blocks)
 {
@@ -565,12 +565,22 @@ __host__ __device__ inline BigLayout big_layout(int L, int L0)
     return B;
 }
 
+// The launch arguments of the big-core kernels: the products, and the
+// workspace layout and the grid of every top depth, computed once on the
+// host (per wave they were a loop of 64-bit divisions on the scalar unit).
+struct BigBatch {
+    BigCore c[kBigBatch];
+    int32_t L, L0;
+    BigLayout Y;
+    BigDepth D[kMaxCoreL];
+};
+
 __device__ __forceinline__ uint32_t counted31(uint32_t e, int r) { return ((e - (uint32_t)r) >> 31) & 1u; }
 
 __global__ void big_init_kernel(BigBatch B)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int K = 1 << B.L, x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= K) return;
     int32_t *keyP = C.ws + Y.keyP, *keyQ = C.ws + Y.keyQ;
@@ -583,7 +593,7 @@ __global__ void big_init_kernel(BigBatch B)
 __global__ __launch_bounds__(256) void big_split_count_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int K = 1 << B.L, bit = B.L - d - 1, y = blockIdx.y;
     const int32_t *key = C.ws + (y ? ((d & 1) ? Y.keyQ2 : Y.keyQ) : ((d & 1) ? Y.keyP2 : Y.keyP));
     const int base = blockIdx.x * kSplitTile + threadIdx.x * 8;
@@ -602,7 +612,7 @@ __global__ __launch_bounds__(256) void big_split_count_kernel(BigBatch B, int d)
 __global__ __launch_bounds__(256) void big_split_scan_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int K = 1 << B.L, s = K >> d, y = blockIdx.y;
     const int bps = s / kSplitTile;
     int32_t *cnt = C.ws + Y.cnt + (size_t)y * (K / kSplitTile) + (size_t)blockIdx.x * bps;
@@ -633,7 +643,7 @@ __global__ __launch_bounds__(256) void big_split_scan_kernel(BigBatch B, int d)
 __global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int K = 1 << B.L, s = K >> d, bit = B.L - d - 1, y = blockIdx.y;
     const int32_t *key = C.ws + (y ? ((d & 1) ? Y.keyQ2 : Y.keyQ) : ((d & 1) ? Y.keyP2 : Y.keyP));
     int32_t *key_new = C.ws + (y ? ((d & 1) ? Y.keyQ : Y.keyQ2) : ((d & 1) ? Y.keyP : Y.keyP2));
@@ -674,7 +684,7 @@ __global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int 
 __global__ __launch_bounds__(kThreads) void big_leaf_kernel(BigBatch B)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int d0 = B.L - B.L0, s0 = 1 << B.L0;
     const size_t v0 = (size_t)blockIdx.x * s0;
     const int32_t *keyP = C.ws + ((d0 & 1) ? Y.keyP2 : Y.keyP) + v0;
@@ -692,7 +702,7 @@ __global__ __launch_bounds__(kThreads) void big_leaf_kernel(BigBatch B)
 __global__ void big_lift_kernel(BigBatch B, int d, int flip)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int K = 1 << B.L, s = K >> d, h = s >> 1;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= K) return;
@@ -717,8 +727,8 @@ __global__ void big_lift_kernel(BigBatch B, int d, int flip)
 __global__ __launch_bounds__(256) void big_hist_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
-    const BigDepth D = big_depth(B.L, d);
+    const BigLayout &Y = B.Y;
+    const BigDepth &D = B.D[d];
     const int K = 1 << B.L, y = blockIdx.y;
     const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI));
     __shared__ uint32_t h[kMaxLines];
@@ -857,8 +867,8 @@ constexpr int kSweepWaves = LCS_SWEEP_WAVES;   // waves (boundaries) per workgro
 __global__ __launch_bounds__(64 * kSweepWaves) void big_sweep_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
-    const BigDepth D = big_depth(B.L, d);
+    const BigLayout &Y = B.Y;
+    const BigDepth &D = B.D[d];
     const int K = 1 << B.L, nodes = K / D.s, y = blockIdx.y;
     // the wave's boundary, made scalar so that its addresses stay in SGPRs
     const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * kSweepWaves + (threadIdx.x >> 6)), lane = threadIdx.x & 63;
@@ -882,9 +892,12 @@ __global__ __launch_bounds__(64 * kSweepWaves) void big_sweep_kernel(BigBatch B,
         return;
     }
     const uint32_t *E = (const uint32_t *)(C.ws + (y ? Y.U : Y.CI)) + (size_t)node * D.s + (size_t)seg * D.w;
-    // w is 512 or a multiple of 1024
-    const int pos = D.w == 512 ? sweep_block<8>(E, D.w, D.s, D.glog, j, Yj - Z, lane)
-                               : sweep_block<16>(E, D.w, D.s, D.glog, j, Yj - Z, lane);
+    // w is a power of two >= 128: the whole block in one batch up to 1024 entries
+    const int need = Yj - Z;
+    const int pos = D.w <= 128   ? sweep_block<2>(E, D.w, D.s, D.glog, j, need, lane)
+                    : D.w <= 256 ? sweep_block<4>(E, D.w, D.s, D.glog, j, need, lane)
+                    : D.w <= 512 ? sweep_block<8>(E, D.w, D.s, D.glog, j, need, lane)
+                                 : sweep_block<16>(E, D.w, D.s, D.glog, j, need, lane);
     if (lane == 0 && pos) *T = seg * D.w + pos;
 }
 
@@ -912,8 +925,8 @@ constexpr size_t walk_lds_bytes(int g) { return (size_t)64 * (2 * g + 3) * sizeo
 __global__ __launch_bounds__(64) void big_walk_kernel(BigBatch B, int d, int flip)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
-    const BigDepth D = big_depth(B.L, d);
+    const BigLayout &Y = B.Y;
+    const BigDepth &D = B.D[d];
     const int K = 1 << B.L, s = D.s, nodes = K / s, g = 1 << D.glog, nb = D.nb;
     const int pieces = 2 * nb - 1;
     const int t = blockIdx.x * 64 + threadIdx.x;
@@ -982,7 +995,7 @@ __global__ __launch_bounds__(64) void big_walk_kernel(BigBatch B, int d, int fli
 __global__ void big_out_kernel(BigBatch B, int flip)
 {
     const BigCore &C = B.c[blockIdx.z];
-    const BigLayout Y = big_layout(B.L, B.L0);
+    const BigLayout &Y = B.Y;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x < C.k) C.R[x] = C.ws[(flip ? Y.Rn : Y.R) + x];
 }
@@ -1252,6 +1265,8 @@ int lcsb_launch_core(const BraidCoreDesc *descs, int count, hipStream_t stream)
         BigBatch B{};
         B.L = big[i].L;
         B.L0 = big_leaf_L(B.L);
+        B.Y = big_layout(B.L, B.L0);
+        for (int d = 0; d < B.L - B.L0; ++d) B.D[d] = big_depth(B.L, d);
         int cnt = 0;
         while (i < big.size() && big[i].L == B.L && cnt < kBigBatch) {
             B.c[cnt++] = BigCore{big[i].P, big[i].Q, big[i].R, big[i].scratch, big[i].k};
