@@ -39,6 +39,11 @@
 namespace lcsb {
 
 constexpr int kThreads = 1024;
+// nodes of at least this many rows also cut their walks at column lines
+#ifndef LCS_COL_LINE_MIN_S
+#define LCS_COL_LINE_MIN_S 512
+#endif
+constexpr int kColLineMinS = LCS_COL_LINE_MIN_S;
 // rows per walked chunk of a node (nodes above 128 chunks take s / 128)
 #ifndef LCS_CORE_CHUNK_ROWS
 #define LCS_CORE_CHUNK_ROWS 8
@@ -86,7 +91,7 @@ __device__ __forceinline__ int wave_incl_scan(int x)
 template <typename UT, typename RT, typename RNT>
 __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT *Rn, const uint16_t *pp,
                                                 const uint16_t *pq, int K, int s, int32_t *bT, int32_t *bD,
-                                                int *hist)
+                                                int32_t *cT, int *hist)
 {
     const int h = s >> 1;
     // lift the children's products into node coordinates
@@ -109,7 +114,6 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     const int rc = s < 64 ? s : (s / 128 > kChunkRows ? s / 128 : kChunkRows);
     const int rc_log = __builtin_ctz(rc);   // rc and s are powers of two
     const int nch = s / rc;            // chunks per node
-    const int total = K / rc;          // chunks at this depth
     // chunk entry points: chunk j of a node covers rows [s-(j+1)rc, s-j rc).
     // A node's first chunk (j = 0) enters at T = 0; its entry is computed in
     // the walk, so only chunks j > 0 (nodes of 64 rows or more, at most
@@ -133,14 +137,15 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
         // columns with z_j (cnt).  The wave's LDS accesses run in order; the
         // fences keep the compiler from moving counter stores, adds and loads
         // across each other.
-        auto range_counts = [&](int v0, int c_begin, int c_end, int &lp0, int &lp1, int &cnt0, int &cnt1) {
+        auto range_counts = [&](const UT *E, int v0, int c_begin, int c_end, int &lp0, int &lp1, int &cnt0,
+                                int &cnt1) {
             hlo[j0] = 0;
             hlo[j1] = 0;
             hhi[j0] = 0;
             hhi[j1] = 0;
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             for (int c0 = c_begin; c0 < c_end; c0 += 64) {
-                const uint16_t e = CI[v0 + c0 + lane];
+                const uint16_t e = E[v0 + c0 + lane];
                 const int bk = (s - 1 - (e & kIdx)) >> rc_log;
                 atomicAdd((e & kLoFlag) ? &hlo[bk] : &hhi[bk], 1);
             }
@@ -159,15 +164,15 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
         };
         // Running Z_j over [c_begin, c_end) in rounds of 64 columns for the
         // open boundaries until Z_j reaches Y_j; T = the column where it does.
-        auto find_T = [&](int v0, int c_begin, int c_end, int Y0, int Y1, int Z0, int Z1, bool open0, bool open1,
-                          int &T0, int &T1) {
+        auto find_T = [&](const UT *E, int v0, int c_begin, int c_end, int Y0, int Y1, int Z0, int Z1, bool open0,
+                          bool open1, int &T0, int &T1) {
             for (int c0 = c_begin; c0 < c_end && __ballot(open0 || open1); c0 += 64) {
                 hlo[j0] = 0;
                 hlo[j1] = 0;
                 hhi[j0] = 0;
                 hhi[j1] = 0;
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                const uint16_t e = CI[v0 + c0 + lane];
+                const uint16_t e = E[v0 + c0 + lane];
                 const bool lo = (e & kLoFlag) != 0;
                 const int bk = (s - 1 - (e & kIdx)) >> rc_log;
                 atomicAdd(lo ? &hlo[bk] : &hhi[bk], 1);
@@ -217,48 +222,59 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
         };
         constexpr int kWaves = kThreads / 64;
         const int nodes = K / s;
-        if (nodes >= kWaves || K < 64 * kWaves) {
-            // one wave per node: Y_j from the node's counts, then one sweep
-            for (int node = wave; node < nodes; node += kWaves) {
-                const int v0 = node * s;
-                int Y0, Y1, c0_, c1_;
-                range_counts(v0, 0, s, Y0, Y1, c0_, c1_);
-                int T0 = 0, T1 = 0;
-                find_T(v0, 0, s, Y0, Y1, 0, 0, j0 >= 1 && j0 < nch && Y0 > 0, j1 < nch && Y1 > 0, T0, T1);
-                if (j0 >= 1 && j0 < nch) entry(node, v0, j0, T0);
-                if (j1 < nch) entry(node, v0, j1, T1);
-            }
-        } else {
-            // W = kWaves / nodes waves per node, each on s / W consecutive
-            // columns: every wave counts its segment, the segments' counts
-            // give Y_j and each segment's starting Z_j, and only the wave in
-            // whose segment Z_j reaches Y_j sweeps for T_j.
-            const int W = kWaves / nodes, node = wave / W, seg = wave % W, seg_len = s / W;
-            const int v0 = node * s, cb = seg * seg_len;
-            int lp0, lp1, cnt0, cnt1;
-            range_counts(v0, cb, cb + seg_len, lp0, lp1, cnt0, cnt1);
-            hlo[j0] = cnt0;
-            hlo[j1] = cnt1;
-            hhi[j0] = lp0;
-            hhi[j1] = lp1;
-            __syncthreads();
-            int Y0 = 0, Y1 = 0, Zs0 = 0, Zs1 = 0;
-            for (int w2 = 0; w2 < W; ++w2) {
-                const int *other = hist + (node * W + w2) * 2 * kMaxChunks;
-                Y0 += other[kMaxChunks + j0];
-                Y1 += other[kMaxChunks + j1];
-                if (w2 < seg) {
-                    Zs0 += other[j0];
-                    Zs1 += other[j1];
+        // dir 0: row lines from the columns' entries (CI); dir 1 (nodes of
+        // kColLineMinS rows or more): column lines from the rows' entries
+        // (U), the column line c = i rc being boundary nch - i, T = rho + 1
+        for (int dir = 0; dir < (s >= kColLineMinS ? 2 : 1); ++dir) {
+            const UT *E = dir ? U : CI;
+            auto store = [&](int node, int v0, int j, int T) {
+                if (dir) cT[node * nch + j] = T;
+                else entry(node, v0, j, T);
+            };
+            if (dir) __syncthreads();   // the row direction's counters are read
+            if (nodes >= kWaves || K < 64 * kWaves) {
+                // one wave per node: Y_j from the node's counts, then one sweep
+                for (int node = wave; node < nodes; node += kWaves) {
+                    const int v0 = node * s;
+                    int Y0, Y1, c0_, c1_;
+                    range_counts(E, v0, 0, s, Y0, Y1, c0_, c1_);
+                    int T0 = 0, T1 = 0;
+                    find_T(E, v0, 0, s, Y0, Y1, 0, 0, j0 >= 1 && j0 < nch && Y0 > 0, j1 < nch && Y1 > 0, T0, T1);
+                    if (j0 >= 1 && j0 < nch) store(node, v0, j0, T0);
+                    if (j1 < nch) store(node, v0, j1, T1);
                 }
+            } else {
+                // W = kWaves / nodes waves per node, each on s / W consecutive
+                // entries: every wave counts its segment, the segments' counts
+                // give Y_j and each segment's starting Z_j, and only the wave in
+                // whose segment Z_j reaches Y_j sweeps for T_j.
+                const int W = kWaves / nodes, node = wave / W, seg = wave % W, seg_len = s / W;
+                const int v0 = node * s, cb = seg * seg_len;
+                int lp0, lp1, cnt0, cnt1;
+                range_counts(E, v0, cb, cb + seg_len, lp0, lp1, cnt0, cnt1);
+                hlo[j0] = cnt0;
+                hlo[j1] = cnt1;
+                hhi[j0] = lp0;
+                hhi[j1] = lp1;
+                __syncthreads();
+                int Y0 = 0, Y1 = 0, Zs0 = 0, Zs1 = 0;
+                for (int w2 = 0; w2 < W; ++w2) {
+                    const int *other = hist + (node * W + w2) * 2 * kMaxChunks;
+                    Y0 += other[kMaxChunks + j0];
+                    Y1 += other[kMaxChunks + j1];
+                    if (w2 < seg) {
+                        Zs0 += other[j0];
+                        Zs1 += other[j1];
+                    }
+                }
+                __syncthreads();   // every wave has read the counts before they are reused
+                const bool mine0 = j0 >= 1 && j0 < nch && Y0 > 0 && Zs0 < Y0 && Y0 <= Zs0 + cnt0;
+                const bool mine1 = j1 < nch && Y1 > 0 && Zs1 < Y1 && Y1 <= Zs1 + cnt1;
+                int T0 = 0, T1 = 0;
+                find_T(E, v0, cb, cb + seg_len, Y0, Y1, Zs0, Zs1, mine0, mine1, T0, T1);
+                if (j0 >= 1 && j0 < nch && (mine0 || (Y0 == 0 && seg == 0))) store(node, v0, j0, T0);
+                if (j1 < nch && (mine1 || (Y1 == 0 && seg == 0))) store(node, v0, j1, T1);
             }
-            __syncthreads();   // every wave has read the counts before they are reused
-            const bool mine0 = j0 >= 1 && j0 < nch && Y0 > 0 && Zs0 < Y0 && Y0 <= Zs0 + cnt0;
-            const bool mine1 = j1 < nch && Y1 > 0 && Zs1 < Y1 && Y1 <= Zs1 + cnt1;
-            int T0 = 0, T1 = 0;
-            find_T(v0, cb, cb + seg_len, Y0, Y1, Zs0, Zs1, mine0, mine1, T0, T1);
-            if (j0 >= 1 && j0 < nch && (mine0 || (Y0 == 0 && seg == 0))) entry(node, v0, j0, T0);
-            if (j1 < nch && (mine1 || (Y1 == 0 && seg == 0))) entry(node, v0, j1, T1);
         }
     }
     __syncthreads();
@@ -271,22 +287,46 @@ __device__ __forceinline__ void bottom_up_depth(UT *U, UT *CI, const RT *R, RNT 
     // counted(e, r) = lo ? idx >= r : idx < r for an entry e = lo << 15 | idx:
     // bit 15 of e - r, for idx in [0, 32768) and r in [0, 32768].
     auto counted = [](uint32_t e, int r) { return (int)(((e - (uint32_t)r) >> 15) & 1u); };
-    for (int ch = threadIdx.x; ch < total; ch += kThreads) {
-        const int v0 = (ch / nch) * s, j = ch % nch;
-        const int r_hi = s - j * rc, r_lo = r_hi - rc;
-        int dr = r_hi - 1, dc, delta;
-        if (j == 0) {
-            // T(s) = 0, delta(s, 0) = 0; entering row s-1 subtracts it when it is lo
-            dc = 0;
-            delta = (U[v0 + s - 1] & kLoFlag) ? -1 : 0;
+    // Pieces: chunk j (q = j < nch) starts at its row line; in nodes of
+    // kColLineMinS rows or more the walks are also cut at the column lines
+    // c = i rc (q = nch - 1 + i), so every piece stays in one rc x rc box
+    // and takes at most 2 rc + 1 steps, whatever the permutations look like
+    // (a strip permutation's long straight runs would otherwise fall to a
+    // few chunks).  Smaller nodes walk whole row chunks.
+    const bool boxes = s >= kColLineMinS && nch > 1;
+    const int per_node = boxes ? 2 * nch - 1 : nch, pieces = (K / s) * per_node;
+    for (int pc = threadIdx.x; pc < pieces; pc += kThreads) {
+        const int node = pc / per_node, q = pc % per_node, v0 = node * s;
+        int dr, dc, delta, r_lo, c_hi;
+        if (q < nch) {
+            const int j = q, r_hi = s - j * rc;
+            r_lo = r_hi - rc;
+            dr = r_hi - 1;
+            if (j == 0) {
+                // T(s) = 0, delta(s, 0) = 0; entering row s-1 subtracts it when it is lo
+                dc = 0;
+                delta = (U[v0 + s - 1] & kLoFlag) ? -1 : 0;
+            } else {
+                dc = min(max(bT[node * nch + j], 0), s);
+                delta = bD[node * nch + j];
+            }
+            const int cb = dc == 0 ? 0 : (dc - 1) >> rc_log;
+            c_hi = !boxes || cb >= nch - 1 ? s + 1 : (cb + 1) * rc + 1;
         } else {
-            dc = bT[ch];
-            delta = bD[ch];
+            // stepping right over column line c = i rc at row rho: delta = -1 + [CI(c) counted at rho]
+            const int i = q - nch + 1;
+            const int Tc = min(max(cT[node * nch + nch - i], 0), s);
+            if (Tc == 0) continue;   // the ant never reaches this column line
+            dr = Tc - 1;
+            dc = i * rc + 1;
+            delta = -1 + counted(CI[v0 + i * rc], dr);
+            r_lo = s - (((s - 1 - dr) >> rc_log) + 1) * rc;
+            c_hi = i == nch - 1 ? s + 1 : (i + 1) * rc + 1;
         }
         // entries outside the node (dc = 0 or s, dr = 0) are read clamped and never used
         uint32_t cl = CI[v0 + max(dc - 1, 0)], cc = CI[v0 + min(dc, s - 1)];
         uint32_t ud = U[v0 + dr], ur = U[v0 + max(dr - 1, 0)];
-        while (dr >= r_lo) {
+        while (dr >= r_lo && dc < c_hi) {
             const int d_left = delta - counted(cl, dr);
             const int d_down = delta + counted(ud, dc);
             if (dc > 0 && d_left < 0 && d_down > 0) Rn[v0 + dr] = dc - 1;
@@ -398,7 +438,7 @@ __device__ __forceinline__ void split_depth_pq(const uint16_t *keyP, uint16_t *k
 // big core reads its slice of the big core's keys).
 template <bool QKEY>
 __device__ __forceinline__ void core_lds_body(const int32_t *P, const int32_t *Q, int32_t *Rout, int k, int L,
-                                              uint16_t *lds, int32_t *bT, int32_t *bD, int *hist)
+                                              uint16_t *lds, int32_t *bT, int32_t *bD, int32_t *cT, int *hist)
 {
     const int K = 1 << L, mask = K - 1;
     uint16_t *keyP = lds, *keyP2 = lds + K;                       // top-down
@@ -429,7 +469,7 @@ __device__ __forceinline__ void core_lds_body(const int32_t *P, const int32_t *Q
     for (int x = threadIdx.x; x < K; x += kThreads) R[x] = 0;  // depth L: 1x1 nodes
     __syncthreads();
     for (int d = L - 1; d >= 0; --d) {
-        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, hist);
+        bottom_up_depth(U, CI, R, Rn, parP + (size_t)d * K, parQ + (size_t)d * K, K, K >> d, bT, bD, cT, hist);
         uint16_t *tmp = R; R = Rn; Rn = tmp;
     }
     for (int x = threadIdx.x; x < k; x += kThreads) Rout[x] = R[x];
@@ -439,9 +479,9 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 {
     const BraidCoreDesc D = batch.d[blockIdx.x];
     extern __shared__ uint16_t lds[];
-    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows], cT[kLdsCoreK / kChunkRows];
     __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
-    core_lds_body<false>(D.P, D.Q, D.R, D.k, D.L, lds, bT, bD, hist);
+    core_lds_body<false>(D.P, D.Q, D.R, D.k, D.L, lds, bT, bD, cT, hist);
 }
 
 // ---------------------------------------------------------------- big cores
@@ -690,10 +730,10 @@ __global__ __launch_bounds__(kThreads) void big_leaf_kernel(BigBatch B)
     const int32_t *keyP = C.ws + ((d0 & 1) ? Y.keyP2 : Y.keyP) + v0;
     const int32_t *keyQ = C.ws + ((d0 & 1) ? Y.keyQ2 : Y.keyQ) + v0;
     extern __shared__ uint16_t lds[];
-    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows];
+    __shared__ int32_t bT[kLdsCoreK / kChunkRows], bD[kLdsCoreK / kChunkRows], cT[kLdsCoreK / kChunkRows];
     __shared__ int hist[(kThreads / 64) * 2 * kMaxChunks];
     // depth d0 of the big core's own recursion: the products go to R
-    core_lds_body<true>(keyP, keyQ, C.ws + Y.R + v0, s0, B.L0, lds, bT, bD, hist);
+    core_lds_body<true>(keyP, keyQ, C.ws + Y.R + v0, s0, B.L0, lds, bT, bD, cT, hist);
 }
 
 // lift the children's products into node coordinates (node size s).  The
