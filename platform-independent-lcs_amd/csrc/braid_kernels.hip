@@ -41,7 +41,7 @@ namespace lcsb {
 constexpr int kThreads = 1024;
 // nodes of at least this many rows also cut their walks at column lines
 #ifndef LCS_COL_LINE_MIN_S
-#define LCS_COL_LINE_MIN_S 512
+#define LCS_COL_LINE_MIN_S 256
 #endif
 constexpr int kColLineMinS = LCS_COL_LINE_MIN_S;
 // rows per walked chunk of a node (nodes above 128 chunks take s / 128)
@@ -516,12 +516,24 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 constexpr int kBigBatch = 16;          // products per big-core launch
 constexpr int kSplitTile = 2048;       // elements per block of a device-wide partition
 constexpr int kMaxLines = 8192;        // grid lines per node and direction
-// box side g of a node's grid (nodes above kMaxLines * g take g = s / kMaxLines)
+// Box side g of a node's grid: kBoxGSmall for nodes up to 2^kBoxSmallMaxL
+// points, kBoxG above (nodes above kMaxLines * g take g = s / kMaxLines).
+// Smaller boxes halve the walks and double the lines whose crossings the
+// sweeps find; the sweeps are cheap for small nodes.  Same box: the Zipf
+// pair's merges (nodes up to 2^14) 1.60 ms with boxes of 32 against 1.78
+// with 64; a k = 2^20 product 1.60 ms with 32 everywhere against 1.47.
 #ifndef LCS_BIG_BOX_G
 #define LCS_BIG_BOX_G 64
 #endif
-constexpr int kBoxG = LCS_BIG_BOX_G;
+#ifndef LCS_BIG_BOX_G_SMALL
+#define LCS_BIG_BOX_G_SMALL 32
+#endif
+#ifndef LCS_BIG_BOX_SMALL_MAX_L
+#define LCS_BIG_BOX_SMALL_MAX_L 14
+#endif
+constexpr int kBoxG = LCS_BIG_BOX_G, kBoxGSmall = LCS_BIG_BOX_G_SMALL, kBoxSmallMaxL = LCS_BIG_BOX_SMALL_MAX_L;
 static_assert(kBoxG >= 16 && kBoxG <= 512 && (kBoxG & (kBoxG - 1)) == 0, "box side");
+static_assert(kBoxGSmall >= 16 && kBoxGSmall <= 512 && (kBoxGSmall & (kBoxGSmall - 1)) == 0, "box side");
 constexpr uint32_t kLo31 = 0x80000000u;
 
 struct BigCore {
@@ -560,7 +572,8 @@ __host__ __device__ inline BigDepth big_depth(int L, int d)
 {
     BigDepth D;
     D.s = 1 << (L - d);
-    D.nb = D.s / kBoxG < kMaxLines ? D.s / kBoxG : kMaxLines;
+    const int g = L - d <= kBoxSmallMaxL ? kBoxGSmall : kBoxG;
+    D.nb = D.s / g < kMaxLines ? D.s / g : kMaxLines;
     D.glog = __builtin_ctz((unsigned)(D.s / D.nb));
     D.w = 1 << big_w_log(L - d);
     if (D.w > D.s) D.w = D.s;
