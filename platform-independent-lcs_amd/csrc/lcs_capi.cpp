@@ -322,6 +322,7 @@ int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, h
 // (the same kernels, same box): one strip 145.0 ms, two 138.0 (chain 136.0
 // + merge 2.0), four 136.7 (131.6 + 5.1), eight 140.3 (129.3 + 11.0).
 constexpr int32_t kDefaultStripMaxM = 32768;
+constexpr double kLargeThinCells = 34359738368.0;   // 2^35
 constexpr int32_t kTallStripMinN = 1 << 18;    // two strips from here
 constexpr int32_t kTallStrip4MinN = 1 << 20;   // four strips from here
 int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
@@ -465,11 +466,19 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
     if (!tm) {
         tm = default_tile_m(ctx, m);
         if (strips_req != 1 && !d_h_in && !d_v_in && tm < 4 && default_strips(ctx, m, n, 4) > 1) {
-            // a thin pair cut into strips: 8 rows per lane when at most 32
-            // strips of them fill every SIMD (fewer merge levels than with 4
-            // rows per lane would cost), else 4
-            const int s8 = default_strips(ctx, m, n, 8);
-            tm = (s8 <= 32 && (long long)s8 * bands_of(m, n, 8) >= 4LL * ctx->num_cus) ? 8 : 4;
+            // a thin pair cut into strips: 16 rows per lane for a large pair
+            // (2^35 cells or more) when at most 64 strips of such bands fill
+            // every SIMD -- on the Zipf pair (m = 2^14, n = 2^22) 64 strips of
+            // 16 rows took 10.64 ms against 10.91 for 32 strips of 8 rows: the
+            // chain's 0.9 ms outweighs a sixth merge level; else 8 rows when
+            // at most 32 strips of them fill every SIMD (fewer merge levels
+            // than with 4 rows per lane would cost), else 4
+            const long long simds = 4LL * ctx->num_cus;
+            const int s16 = default_strips(ctx, m, n, 16), s8 = default_strips(ctx, m, n, 8);
+            if ((double)m * n >= kLargeThinCells && s16 <= 64 && (long long)s16 * bands_of(m, n, 16) >= simds)
+                tm = 16;
+            else
+                tm = (s8 <= 32 && (long long)s8 * bands_of(m, n, 8) >= simds) ? 8 : 4;
         }
     }
     int strips = strips_req ? strips_req : ((d_h_in || d_v_in) ? 1 : default_strips(ctx, m, n, tm));
