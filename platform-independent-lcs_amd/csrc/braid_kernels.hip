@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 //    it stays inside one g x g box and takes at most 2g + 1 steps.
 // Entries of U (row -> column) and CI (column -> row) are node-local
 // indices with the lo flag in bit 31.
-constexpr int kBigBatch = 16;          // products per big-core launch
+constexpr int kBigBatch = 32;          // products per big-core launch
 constexpr int kSplitTile = 2048;       // elements per block of a device-wide partition
 constexpr int kMaxLines = 8192;        // grid lines per node and direction
 // Box side g of a node's grid: kBoxGSmall for nodes up to 2^kBoxSmallMaxL
