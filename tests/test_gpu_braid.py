@@ -34,6 +34,24 @@ def test_core_sizes(ctx, k):
     assert np.array_equal(ctx.braid_multiply(p, q, k, False), oracle.staggered_multiply(p, q, k, False))
 
 
+@pytest.mark.parametrize("k", [300, 2048])
+def test_core_structured_permutations(ctx, k):
+    """One-workgroup cores on permutations whose ant paths run straight for
+    long stretches (identity, reversal, shifts, few swaps): the walks cut at
+    row and column lines must still join up."""
+    rng = np.random.default_rng(k)
+    ident = np.arange(k, dtype=np.int32)
+    near = ident.copy()
+    for _ in range(20):
+        i, j = rng.integers(0, k, 2)
+        near[i], near[j] = near[j], near[i]
+    cases = [(ident, ident[::-1].copy()), (ident[::-1].copy(), ident[::-1].copy()), (np.roll(ident, 37), near),
+             (near, rng.permutation(k).astype(np.int32)), (ident[::-1].copy(), ident)]
+    for p, q in cases:
+        for glue in (False, True):
+            assert np.array_equal(ctx.braid_multiply(p, q, k, glue), oracle.staggered_multiply(p, q, k, glue))
+
+
 def test_glued_products_of_combs(ctx):
     rng = np.random.default_rng(7)
     for t in range(40):
