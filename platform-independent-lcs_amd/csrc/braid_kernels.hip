@@ -19,10 +19,11 @@
 //    chunks of rows: the dominance-sum difference
 //        delta(r, c) = #{hi points: row < r, col < c} - #{lo points: row >= r, col >= c}
 //    is non-decreasing in r and c, so the ant leaves row r at the first
-//    column T(r) where delta >= 0.  One wave finds T and delta at each chunk
-//    boundary with a single pass over the node's columns; then one thread
-//    per chunk walks it with the reference's incremental updates and places
-//    the good points.
+//    column T(r) where delta >= 0.  Waves find T and delta at each chunk
+//    boundary with sweeps over the node's columns (and, in nodes of 256
+//    rows or more, where the ant crosses every rc-th column with sweeps
+//    over its rows); then one thread per piece walks it with the
+//    reference's incremental updates and places the good points.
 // The union of the children's points, the per-column view and the parents
 // of every depth's partition live in LDS (16-bit entries, the lo/hi flag in
 // bit 15), so the walks and the lifts run at LDS latency.  Cores up to
@@ -485,14 +486,15 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 }
 
 // ---------------------------------------------------------------- big cores
-// Cores above kLdsCoreK run the same recursion spread over the whole GPU:
+// Cores above kLdsCoreK run the same recursion spread over the whole GPU
+// (the one-workgroup core walks its top depths in the same box pieces):
 //  * the top d0 = L - L0 depths of the top-down split are device-wide
 //    stable partitions (count, scan, scatter per depth);
 //  * the 2^d0 nodes of depth d0 are independent sticky products of 2^L0
 //    points each, one workgroup per node (core_lds_body on their keys);
 //  * bottom-up, each top depth lifts the children's products into node
 //    coordinates (inverse_mapping, src/braid_multiplication.hpp:191-202)
-//    and runs the ant passage (:123-189) cut into pieces that a wave each
+//    and runs the ant passage (:123-189) cut into pieces that a lane each
 //    walks in LDS.  The ant leaves row r at T(r) = min{c : delta(r, c) >= 0}
 //    (delta non-decreasing in r and c), so its path is fixed by where it
 //    crosses a grid of lines every g rows and every g columns of a node:
@@ -507,10 +509,11 @@ __global__ __launch_bounds__(kThreads) void braid_core_lds_kernel(CoreBatch batc
 //        z_j(e) = lo(e) ? b(e) < j : b(e) >= j,  Y_j = #{lo entries: b < j},
 //        T_j = min{x : #{x' < x : z_j(e_x')} >= Y_j},
 //    the column line c = i g being boundary j = s/g - i over the rows
-//    (T_j = rho + 1).  Segment histograms, a scan over the segments per
-//    boundary and a sweep of the one segment where the count crosses find
-//    every T_j.  A piece starts at a crossing and ends at the next one, so
-//    it stays inside one g x g box and takes at most 2g + 1 steps.
+//    (T_j = rho + 1).  Block histograms (big_hist_kernel), a search over
+//    the blocks per boundary and a sweep of the one block where the count
+//    crosses (big_sweep_kernel) find every T_j.  A piece starts at a
+//    crossing and ends at the next one, so it stays inside one g x g box and
+//    takes at most 2g + 1 steps (big_walk_kernel).
 // Entries of U (row -> column) and CI (column -> row) are node-local
 // indices with the lo flag in bit 31.
 constexpr int kBigBatch = 32;          // products per big-core launch
