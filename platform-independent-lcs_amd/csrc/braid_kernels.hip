@@ -586,7 +586,7 @@ __host__ __device__ inline BigDepth big_depth(int L, int d)
 
 // word offsets inside one product's workspace
 struct BigLayout {
-    size_t keyP, keyQ, keyP2, keyQ2, parP, parQ, R, Rn, U, CI, cnt, segtot, hist, T, total;
+    size_t keyP, keyQ, keyP2, keyQ2, parP, parQ, R, Rn, U, CI, cnt, hist, T, total;
 };
 
 __host__ __device__ inline BigLayout big_layout(int L, int L0)
@@ -605,8 +605,7 @@ __host__ __device__ inline BigLayout big_layout(int L, int L0)
     B.Rn = o; o += K;
     B.U = o; o += K;
     B.CI = o; o += K;
-    B.cnt = o; o += 2 * (K / kSplitTile);
-    B.segtot = o; o += 2 * ((size_t)1 << d0);
+    B.cnt = o; o += 3 * 2 * (K / kSplitTile);   // tile counts of P and Q, three depths in rotation
     size_t hist = 0, lines = 0;
     for (int d = 0; d < d0; ++d) {
         const BigDepth D = big_depth(L, d);
@@ -643,9 +642,15 @@ __global__ void big_init_kernel(BigBatch B)
     keyP[x] = x < C.k ? C.P[x] : x;
     if (x >= C.k) keyQ[x] = x;
     else if ((uint32_t)C.Q[x] < (uint32_t)C.k) keyQ[C.Q[x]] = x;
+    // the tile counts of depths 1 and 2 are accumulated by the scatters
+    // before them and start at zero
+    const int ntiles = K / kSplitTile;
+    if (x < 2 * 2 * ntiles) C.ws[Y.cnt + 2 * (size_t)ntiles + x] = 0;
 }
 
-// ones of bit `bit` per tile of kSplitTile keys (y = 0: P keys, 1: Q keys)
+// Depth-0 ones of the top bit per tile of kSplitTile keys (y = 0: P keys,
+// 1: Q keys), into count buffer 0; every later depth's counts are
+// accumulated by the scatter of the depth before.
 __global__ __launch_bounds__(256) void big_split_count_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
@@ -660,50 +665,30 @@ __global__ __launch_bounds__(256) void big_split_count_kernel(BigBatch B, int d)
     __shared__ int red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) C.ws[Y.cnt + (size_t)y * (K / kSplitTile) + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0)
+        C.ws[Y.cnt + ((size_t)(d % 3) * 2 + y) * (K / kSplitTile) + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// exclusive scan of the tile counts of segment blockIdx.x (size s) in place;
-// the segment's total ones to segtot
-__global__ __launch_bounds__(256) void big_split_scan_kernel(BigBatch B, int d)
-{
-    const BigCore &C = B.c[blockIdx.z];
-    const BigLayout &Y = B.Y;
-    const int K = 1 << B.L, s = K >> d, y = blockIdx.y;
-    const int bps = s / kSplitTile;
-    int32_t *cnt = C.ws + Y.cnt + (size_t)y * (K / kSplitTile) + (size_t)blockIdx.x * bps;
-    const int per = (bps + 255) / 256;
-    const int lo = min((int)threadIdx.x * per, bps), hi = min(lo + per, bps);
-    int run = 0;
-    for (int i = lo; i < hi; ++i) run += cnt[i];
-    __shared__ int sums[256];
-    sums[threadIdx.x] = run;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int v = (int)threadIdx.x >= off ? sums[threadIdx.x - off] : 0;
-        __syncthreads();
-        sums[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int acc = sums[threadIdx.x] - run;
-    for (int i = lo; i < hi; ++i) {
-        const int c = cnt[i];
-        cnt[i] = acc;
-        acc += c;
-    }
-    if (threadIdx.x == 255) C.ws[Y.segtot + (size_t)y * ((size_t)1 << (B.L - B.L0)) + blockIdx.x] = sums[255];
-}
-
-// stable partition of every segment of size s by bit `bit` (zeros first);
-// par[pos] = old position
+// Stable partition of every segment of size s by bit `bit` (zeros first);
+// par[pos] = old position.  The block's own tile count prefix and its
+// segment's total ones come from the tile counts of this depth (count
+// buffer d % 3), summed by the block itself.  When another device-wide
+// depth follows (NEXT), the block also counts the next bit's ones of the
+// keys it places per destination tile -- its zeros and its ones each land
+// in a run of at most kSplitTile positions, so at most two tiles each --
+// and adds them to buffer (d + 1) % 3, and clears its own tile in buffer
+// (d + 2) % 3 for the depth after that.
+template <bool NEXT>
 __global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int d)
 {
     const BigCore &C = B.c[blockIdx.z];
     const BigLayout &Y = B.Y;
     const int K = 1 << B.L, s = K >> d, bit = B.L - d - 1, y = blockIdx.y;
+    const int ntiles = K / kSplitTile;
     const int32_t *key = C.ws + (y ? ((d & 1) ? Y.keyQ2 : Y.keyQ) : ((d & 1) ? Y.keyP2 : Y.keyP));
     int32_t *key_new = C.ws + (y ? ((d & 1) ? Y.keyQ : Y.keyQ2) : ((d & 1) ? Y.keyP : Y.keyP2));
     int32_t *par = C.ws + (y ? Y.parQ : Y.parP) + (size_t)d * K;
+    const int32_t *cnt = C.ws + Y.cnt + ((size_t)(d % 3) * 2 + y) * ntiles;
     const int base = blockIdx.x * kSplitTile + threadIdx.x * 8;
     int32_t kv[8];
     int c = 0;
@@ -712,19 +697,43 @@ __global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int 
         kv[q] = key[base + q];
         c += (kv[q] >> bit) & 1;
     }
-    // exclusive prefix of the thread counts over the block
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the segment's tiles t0 .. t0+nt-1; ones in the tiles before this one and in all of them
+    const int seg = (blockIdx.x * kSplitTile) & ~(s - 1);
+    const int t0 = seg / kSplitTile, nt = s / kSplitTile, mine = blockIdx.x - t0;
+    int pre = 0, all = 0;
+    for (int i = threadIdx.x; i < nt; i += 256) {
+        const int v = cnt[t0 + i];
+        all += v;
+        pre += i < mine ? v : 0;
+    }
+    // exclusive prefix of the thread counts over the block, and the block sums
     const int incl = wave_incl_scan(c);
-    __shared__ int wsum[4];
+    for (int off = 32; off > 0; off >>= 1) {
+        pre += __shfl_xor(pre, off);
+        all += __shfl_xor(all, off);
+    }
+    __shared__ int wsum[4], wpre[4], wall[4];
+    __shared__ int nz[2], no[2];
     if (lane == 63) wsum[wave] = incl;
+    if (lane == 0) {
+        wpre[wave] = pre;
+        wall[wave] = all;
+    }
+    if (NEXT && threadIdx.x < 2) {
+        nz[threadIdx.x] = 0;
+        no[threadIdx.x] = 0;
+    }
     __syncthreads();
     int before = incl - c;
     for (int w2 = 0; w2 < wave; ++w2) before += wsum[w2];
-    const int seg = (blockIdx.x * kSplitTile) & ~(s - 1);
-    const int segi = seg / s;
-    before += C.ws[Y.cnt + (size_t)y * (K / kSplitTile) + blockIdx.x];
-    const int ones_total = C.ws[Y.segtot + (size_t)y * ((size_t)1 << (B.L - B.L0)) + segi];
+    const int before_tiles = wpre[0] + wpre[1] + wpre[2] + wpre[3];
+    const int ones_total = wall[0] + wall[1] + wall[2] + wall[3];
+    before += before_tiles;
     const int zeros_total = s - ones_total;
+    // first destinations of the block's zeros and ones (their runs' first tiles)
+    const int z0 = blockIdx.x * kSplitTile - before_tiles, o0 = seg + zeros_total + before_tiles;
+    const int zt = z0 / kSplitTile, ot = o0 / kSplitTile;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int x = base + q;
@@ -733,6 +742,18 @@ __global__ __launch_bounds__(256) void big_split_scatter_kernel(BigBatch B, int 
         before += b;
         key_new[pos] = kv[q];
         par[pos] = x;
+        if (NEXT && ((kv[q] >> (bit - 1)) & 1))
+            atomicAdd(b ? &no[pos / kSplitTile - ot] : &nz[pos / kSplitTile - zt], 1);
+    }
+    if (NEXT) {
+        int32_t *next = C.ws + Y.cnt + ((size_t)((d + 1) % 3) * 2 + y) * ntiles;
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            const int v = threadIdx.x < 2 ? nz[threadIdx.x] : no[threadIdx.x - 2];
+            const int tile = (threadIdx.x < 2 ? zt : ot) + (threadIdx.x & 1);
+            if (v) atomicAdd(&next[tile], v);
+        }
+        if (threadIdx.x == 0) C.ws[Y.cnt + ((size_t)((d + 2) % 3) * 2 + y) * ntiles + blockIdx.x] = 0;
     }
 }
 
@@ -1252,10 +1273,10 @@ static int launch_big(const BigBatch &B, int cnt, hipStream_t stream)
     const int L = B.L, L0 = B.L0, d0 = L - L0, K = 1 << L;
     const dim3 lin((unsigned)(K / 256), 1, (unsigned)cnt), tiles((unsigned)(K / kSplitTile), 2, (unsigned)cnt);
     hipLaunchKernelGGL(big_init_kernel, lin, dim3(256), 0, stream, B);
+    hipLaunchKernelGGL(big_split_count_kernel, tiles, dim3(256), 0, stream, B, 0);
     for (int d = 0; d < d0; ++d) {
-        hipLaunchKernelGGL(big_split_count_kernel, tiles, dim3(256), 0, stream, B, d);
-        hipLaunchKernelGGL(big_split_scan_kernel, dim3(1u << d, 2, (unsigned)cnt), dim3(256), 0, stream, B, d);
-        hipLaunchKernelGGL(big_split_scatter_kernel, tiles, dim3(256), 0, stream, B, d);
+        if (d + 1 < d0) hipLaunchKernelGGL(big_split_scatter_kernel<true>, tiles, dim3(256), 0, stream, B, d);
+        else hipLaunchKernelGGL(big_split_scatter_kernel<false>, tiles, dim3(256), 0, stream, B, d);
     }
     hipLaunchKernelGGL(big_leaf_kernel, dim3(1u << d0, 1, (unsigned)cnt), dim3(kThreads), lds_core_bytes(L0),
                        stream, B);

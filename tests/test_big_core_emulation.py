@@ -1,6 +1,6 @@
 """CPU emulation of the whole-GPU steady ant's index arithmetic
 (platform-independent-lcs_amd/csrc/braid_kernels.hip, the "big cores" kernels:
-big_split_*, big_lift_kernel, big_hist_kernel, block_search, sweep_block,
+big_split_count/scatter, big_lift_kernel, big_hist_kernel, block_search, sweep_block,
 big_walk_kernel), step for step in Python, checked against the oracle's
 steady ant (orc_steady_ant, braid_multiplication.hpp:204-270).  The leaves
 (one workgroup each on the GPU) are taken from the oracle here; the GPU tests
@@ -36,27 +36,48 @@ def depth(L, d, box_g, max_lines, w_fixed):
     return s, nb, glog, w, s // w
 
 
-def partition(key, s, bit, tile=2048):
-    """big_split_count/scan/scatter: stable partition of every segment of
-    size s by `bit`, zeros first; par[pos] = old position."""
+def partition(key, s, bit, cnt, tile=2048, next_bit=None):
+    """big_split_scatter_kernel: stable partition of every segment of size s
+    by `bit`, zeros first; par[pos] = old position.  cnt holds this depth's
+    ones per tile; a block sums its segment's tiles itself.  With next_bit,
+    it also returns the next depth's tile counts as the scatter accumulates
+    them: each block counts the next bit of the keys it places per
+    destination tile, its zeros and its ones each running into at most two
+    tiles from their first destinations."""
     K = key.size
     new = np.empty_like(key)
     par = np.empty(K, np.int64)
-    cnt = [int(((key[b * tile:(b + 1) * tile] >> bit) & 1).sum()) for b in range(K // tile)]
-    bps = s // tile
-    for sg in range(K // s):
-        c = cnt[sg * bps:(sg + 1) * bps]
-        ones = sum(c)
-        before = 0
-        for blk in range(sg * bps, (sg + 1) * bps):
-            seg = sg * s
-            for x in range(blk * tile, (blk + 1) * tile):
-                b = (int(key[x]) >> bit) & 1
-                pos = seg + (s - ones) + before if b else seg + (x - seg - before)
-                before += b
-                new[pos] = key[x]
-                par[pos] = x
-    return new, par
+    nxt = np.zeros(K // tile, np.int64)
+    for blk in range(K // tile):
+        seg = (blk * tile) & ~(s - 1)
+        t0, nt = seg // tile, s // tile
+        before_tiles = int(sum(cnt[t0:t0 + (blk - t0)]))
+        ones_total = int(sum(cnt[t0:t0 + nt]))
+        zeros_total = s - ones_total
+        z0, o0 = blk * tile - before_tiles, seg + zeros_total + before_tiles
+        zt, ot = z0 // tile, o0 // tile
+        runs = {("z", 0): 0, ("z", 1): 0, ("o", 0): 0, ("o", 1): 0}
+        before = before_tiles
+        for x in range(blk * tile, (blk + 1) * tile):
+            b = (int(key[x]) >> bit) & 1
+            pos = seg + zeros_total + before if b else seg + (x - seg - before)
+            before += b
+            new[pos] = key[x]
+            par[pos] = x
+            if next_bit is not None and (int(key[x]) >> next_bit) & 1:
+                idx = pos // tile - (ot if b else zt)
+                assert idx in (0, 1)
+                runs[("o" if b else "z", idx)] += 1
+        if next_bit is not None:
+            for (kind, idx), v in runs.items():
+                if v:
+                    nxt[(ot if kind == "o" else zt) + idx] += v
+    return new, par, nxt
+
+
+def tile_counts(key, bit, tile=2048):
+    """big_split_count_kernel (depth 0): ones of `bit` per tile."""
+    return np.array([int(((key[b * tile:(b + 1) * tile] >> bit) & 1).sum()) for b in range(key.size // tile)])
 
 
 def block_search(Hj, Ht, nsegs, Yj):
@@ -85,9 +106,15 @@ def emulate(P, Q, k, L, L0, box_g=64, max_lines=8192, w_fixed=None):
     for r in range(k):
         keyQ[Q[r]] = r
     parP, parQ = [], []
+    cP, cQ = tile_counts(keyP, L - 1), tile_counts(keyQ, L - 1)
     for d in range(d0):
-        keyP, pp = partition(keyP, K >> d, L - d - 1)
-        keyQ, pq = partition(keyQ, K >> d, L - d - 1)
+        nb = L - d - 2 if d + 1 < d0 else None
+        keyP, pp, nP = partition(keyP, K >> d, L - d - 1, cP, next_bit=nb)
+        keyQ, pq, nQ = partition(keyQ, K >> d, L - d - 1, cQ, next_bit=nb)
+        if nb is not None:
+            # the accumulated counts are the next depth's tile counts
+            assert np.array_equal(nP, tile_counts(keyP, nb)) and np.array_equal(nQ, tile_counts(keyQ, nb))
+            cP, cQ = nP, nQ
         parP.append(pp)
         parQ.append(pq)
     s0 = 1 << L0
@@ -216,6 +243,7 @@ CASES = [
     (6000, 13, 11, 32, 8192, None),     # boxes of 32
     (4096, 12, 10, 64, 16, 512),        # wide boxes (g = 256 at the top) and 8 blocks per node
     (8192, 13, 10, 16, 8192, 512),      # boxes of 16, 16 blocks per node
+    (16384, 14, 10, 32, 8192, None),    # four device-wide depths, 8 partition tiles per segment
 ]
 
 
