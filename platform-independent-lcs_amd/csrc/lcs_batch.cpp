@@ -298,17 +298,24 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
             hipStreamSynchronize(s) != hipSuccess)
             return LCS_EIO;
         for (int c = 0; c < kNumTiles; ++c) {
-            std::vector<int32_t> mine;
+            // candidates of this tile height ordered by (bands, columns, pair):
+            // one 64-bit key each, so the sort compares integers
+            const int tm = kTileClasses[c];
+            std::vector<unsigned long long> keyed;
             for (size_t i = 0; i < cand.size(); ++i) {
                 if (cand_tm[i] != c) continue;
-                if (bad[i]) groups[(size_t)c].push_back(desc(cand[i]));
-                else mine.push_back(cand[i]);
+                const int32_t p = cand[i];
+                if (bad[i]) {
+                    groups[(size_t)c].push_back(desc(p));
+                    continue;
+                }
+                // bands < 2^10 and n <= 65535 (packable), p < 2^31
+                const unsigned long long bands = (unsigned long long)bands_of(m[p], n[p], tm);
+                keyed.push_back(bands << 48 | (unsigned long long)n[p] << 32 | (uint32_t)p);
             }
-            const int tm = kTileClasses[c];
-            std::sort(mine.begin(), mine.end(), [&](int32_t x, int32_t y) {
-                const int32_t bx = bands_of(m[x], n[x], tm), by = bands_of(m[y], n[y], tm);
-                return bx != by ? bx < by : n[x] < n[y];
-            });
+            std::sort(keyed.begin(), keyed.end());
+            std::vector<int32_t> mine(keyed.size());
+            for (size_t i = 0; i < keyed.size(); ++i) mine[i] = (int32_t)(uint32_t)keyed[i];
             size_t i = 0;
             for (; i + 1 < mine.size(); i += 2) {
                 CoupleDesc D{};
