@@ -14,8 +14,10 @@
 // TM rows (symbols and h strands) in VGPRs.  At step t lane L combs column
 // t-L, so the wave sweeps the band along anti-diagonals.  The v strand and the
 // b symbol move one lane down per step by DPP wave_shr:1; lane 0 receives the
-// band's inputs from two "conveyor" registers shifted by wave_shl:1, and the v
-// strand leaving lane 63 is caught by a third conveyor.  Nothing touches LDS.
+// band's inputs from two "conveyor" registers shifted by wave_shl:1, and the
+// strand conveyor also catches the v strand leaving lane 63: a chunk's C
+// inputs drain out at lane 0 while its C outputs fill lanes 64-C .. 63.
+// Nothing touches LDS.
 // Every C = 32 steps (a chunk) the wave publishes the C newest bottom-edge
 // strands of its band to a ring in global memory as 64-bit (tag, strand) words stored with
 // device-scope relaxed atomics; the band below polls for words carrying its
@@ -43,10 +45,6 @@ constexpr int kWave = 64;
 constexpr int kDppWaveShl1 = 0x130;  // lane L reads lane L+1
 constexpr int kDppWaveShr1 = 0x138;  // lane L reads lane L-1
 
-__device__ __forceinline__ uint32_t shl1_zero(uint32_t x)
-{
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, kDppWaveShl1, 0xf, 0xf, true);
-}
 __device__ __forceinline__ int32_t shl1_zero_i(int32_t x)
 {
     return __builtin_amdgcn_mov_dpp(x, kDppWaveShl1, 0xf, 0xf, true);
@@ -156,20 +154,22 @@ __device__ __forceinline__ void cell16(uint32_t sa, uint32_t sb, uint32_t &h, ui
 }
 
 // Steps t0 .. t0+C-1 of a band.  COLMASK: some lane's column leaves [0, n).
-// The conveyors I and IB come back shifted by C lanes.
+// Q (strands) and IB (symbols) are the conveyors.  On entry Q's lanes
+// 0 .. C-1 hold the chunk's inputs (the input of step t0+s in lane s); each
+// step hands Q's lane 0 to the band and shifts Q down one lane while lane
+// 63's strand enters at the top, so on return Q's lanes 64-C .. 63 hold the
+// chunk's outputs (columns t0-64 .. t0-64+C-1).  One shift serves both
+// directions: C <= 64, so the inputs and outputs never meet.
 template <int TM, int C, bool ROWMASK, bool COLMASK, bool PK = false>
 __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
-                                           uint32_t &v, int32_t &bq, uint32_t &O, uint32_t &I, int32_t &IB,
-                                           int t0, int lane, int n)
+                                           uint32_t &v, int32_t &bq, uint32_t &Q, int32_t &IB, int t0, int lane, int n)
 {
 #pragma unroll
     for (int s = 0; s < C; ++s) {
-        const uint32_t vin = I;
-        I = shl1_zero(I);
         const int32_t bin = IB;
         IB = shl1_zero_i(IB);
-        const uint32_t vn = shr1_keep(vin, v);   // lane 0 takes the band input
-        O = shl1_keep(v, O);                     // lane 63's last output joins O
+        const uint32_t vn = shr1_keep(Q, v);   // lane 0 takes the band input Q[0]
+        Q = shl1_keep(v, Q);                   // inputs move toward lane 0; lane 63's output enters at the top
         v = vn;
         bq = shr1_keep_i(bin, bq);
         bool active = true;
@@ -317,9 +317,9 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
         I = (uint32_t)e;
         return true;
     };
-    uint32_t v = 0, O = 0, Osave = 0, I = 0;
+    uint32_t v = 0, Q = 0, Osave = 0;
     int32_t bq = 0, IB = pB;
-    if (!take_inputs(0, I, std::false_type{})) return false;
+    if (!take_inputs(0, Q, std::false_type{})) return false;
     // One chunk: prefetch chunk k+1, then publish chunk k-1 (the wait for the
     // loads at the end then leaves the store in flight), comb C steps, take
     // the next inputs.  The chunks run in three phases (ramp-in, interior,
@@ -343,11 +343,11 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C, std::integral_constant<bool, !decltype(colmask)::value>{});
-        band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
-        Osave = O;
+        band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, Q, IB, t0, lane, n);
+        Osave = Q;
         if (!decltype(colmask)::value || k + 1 < nchunks) {   // an interior chunk is never the last
             IB = pB;
-            if (!take_inputs(t0 + C, I, std::integral_constant<bool, !decltype(colmask)::value>{})) return false;
+            if (!take_inputs(t0 + C, Q, std::integral_constant<bool, !decltype(colmask)::value>{})) return false;
         }
         return true;
     };
@@ -564,17 +564,17 @@ __device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, 
     };
     int32_t cn = 0;   // consumer word of wave w+1, read a chunk ahead
 
-    uint32_t v = 0, O = 0, Osave = 0, I = 0;
+    uint32_t v = 0, Q = 0, Osave = 0;
     int32_t bq = 0, IB = 0, pB = 0;
     if constexpr (first) {
         pB = bsym[min(ql, nm1)];
         pE = ld_relaxed(Ein + min(ql, nm1));
-        if (!take_inputs(0, I)) return false;
+        if (!take_inputs(0, Q)) return false;
         IB = pB;
         if (lane < C) bring[lane] = pB;
     } else {
         lds_prefetch(0);
-        if (!lds_take(0, I, IB)) return false;
+        if (!lds_take(0, Q, IB)) return false;
     }
     for (int j = 0; j < nchunks; ++j) {
         const int t0 = j * C;
@@ -594,31 +594,31 @@ __device__ bool group_band(const ChainArgs &A, const PairDesc &P, int g, int w, 
         }
         if constexpr (!last) cn = G->cons[w + 1];
         if (t0 >= kWave - 1 && t0 + C <= n)
-            band_steps<TM, C, ROWMASK, false>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
+            band_steps<TM, C, ROWMASK, false>(a, h, valid, v, bq, Q, IB, t0, lane, n);
         else
-            band_steps<TM, C, ROWMASK, true>(a, h, valid, v, bq, O, I, IB, t0, lane, n);
-        Osave = O;
+            band_steps<TM, C, ROWMASK, true>(a, h, valid, v, bq, Q, IB, t0, lane, n);
+        Osave = Q;
         if constexpr (!last) {
             if (j >= kShift && j - kShift < nin) {
                 const int jo = j - kShift;   // the column chunk O now completes
                 if (__builtin_amdgcn_readfirstlane(cn) < jo - kXferSlots &&
                     !lds_wait_ge(&G->cons[w + 1], jo - kXferSlots, G, A))
                     return false;
-                if (lane >= kWave - C) xfer_out[(jo & (kXferSlots - 1)) * C + lane - (kWave - C)] = O;
+                if (lane >= kWave - C) xfer_out[(jo & (kXferSlots - 1)) * C + lane - (kWave - C)] = Osave;
                 if (lane == 0) G->prog[w] = jo;
             }
         }
         if constexpr (first) {
             if (j + 1 < nchunks) {
                 IB = pB;
-                if (!take_inputs(t0 + C, I)) return false;
+                if (!take_inputs(t0 + C, Q)) return false;
                 if (lane < C) bring[((j + 1) & (kBRing - 1)) * C + lane] = pB;
             }
         } else {
             if (j + 1 < nin) {
-                if (!lds_take(j + 1, I, IB)) return false;
+                if (!lds_take(j + 1, Q, IB)) return false;
             } else {
-                I = 0;   // columns past n: no input
+                Q = 0;   // columns past n: no input
                 IB = 0;
             }
         }
