@@ -16,7 +16,23 @@ struct PairDesc {
     int32_t *v_out;              // n strands
     unsigned long long *E;       // 2*n ring words (tag << 32 | strand) + 64 trash words
     int32_t m, n, nbands, tile_m;
+    // Nullable: b as byte symbols (lcsk_launch_sym8), read instead of b by
+    // the register chain when *sym8_bad is 0 (every symbol in [0, 255]).
+    // sym8_col0 is the column of b[0] within those bytes (a strip's offset).
+    const uint32_t *sym8;
+    const int32_t *sym8_bad;
+    int32_t sym8_col0, sym8_stride;
 };
+
+// Byte symbols of a string b of n symbols: four copies (stride words each,
+// lcsk_sym8_stride(n)), copy j holding at byte x the symbol of column
+// x + j - kSym8Pad as a byte (0 outside [0, n)).  A lane whose C-step
+// window starts at byte x0 reads copy x0 % 4 from a word boundary.
+constexpr int32_t kSym8Pad = 128;
+inline int32_t lcsk_sym8_stride(int32_t n) { return (int32_t)((((int64_t)n + 2 * kSym8Pad + 15) / 16) * 4); }
+// Writes the four copies of b into `out` (4 * stride words) and sets *bad
+// (zeroed first) to nonzero when a symbol of b is outside [0, 255].
+int lcsk_launch_sym8(const int32_t *b, int32_t n, uint32_t *out, int32_t stride, int32_t *bad, hipStream_t stream);
 
 // Two pairs combed together by the 16-bit packed chain
 // (LCS_VARIANT_PACKED16): pair q (0 = X, low halves; 1 = Y, high halves)

@@ -73,6 +73,7 @@ __device__ __forceinline__ uint32_t shl1_keep(uint32_t old_, uint32_t x)
 typedef LCSK_GLOBAL unsigned long long gword;
 typedef LCSK_GLOBAL const unsigned long long gcword;
 typedef LCSK_GLOBAL const int32_t gsym;
+typedef LCSK_GLOBAL const uint32_t gsym8;   // a word of four byte symbols
 typedef LCSK_GLOBAL int32_t gout;
 // LDS words polled through a pointer: the explicit address space keeps the
 // reads ds_read (a volatile generic pointer would be a flat access, and a
@@ -188,6 +189,102 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
     }
 }
 
+// All TM rows of one column for one lane when the column symbol is byte SEL
+// of `word` (byte symbols; a row symbol matches it zero-extended, so row
+// symbols outside [0, 255] never match, as no column symbol is).
+template <int TM, bool ROWMASK, int SEL>
+__device__ __forceinline__ void column_cells8(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
+                                              uint32_t word, uint32_t &v)
+{
+    if constexpr (!ROWMASK && TM >= 4) {
+        constexpr int R = TM >= 16 ? 16 : TM;
+#pragma unroll
+        for (int k = 0; k < TM; k += R) {
+            uint32_t hn[R];
+            cell_block_byte<R, SEL>(&a[k], &h[k], hn, (int32_t)word, v);
+#pragma unroll
+            for (int q = 0; q < R; ++q) h[k + q] = hn[q];
+        }
+    } else {
+        const int32_t sb = (int32_t)((word >> (8 * SEL)) & 0xffu);
+#pragma unroll
+        for (int k = 0; k < TM; ++k) cell<ROWMASK>(a[k], sb, valid[k], h[k], v);
+    }
+}
+
+// Step t of a band with byte symbols: the strand moves as in band_steps;
+// the column symbol is byte SEL of `word`, which the lane holds itself.
+// Unmasked steps run as one asm block per step (cell_asm.h: the strand
+// hand-off sits behind the step's first two compares, so the DPP read of v
+// needs no s_nop after the previous step's last cell wrote it).
+template <int TM, bool ROWMASK, bool COLMASK, int SEL>
+__device__ __forceinline__ void band_step8(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
+                                           uint32_t &v, uint32_t &Q, uint32_t word, int t, int lane, int n)
+{
+    if constexpr (!ROWMASK && !COLMASK && TM >= 4) {
+        constexpr int R = TM >= 16 ? 16 : TM;
+        uint32_t w = v, vt;
+        {
+            uint32_t hn[R];
+            cell_step_byte<R, SEL>(&a[0], &h[0], hn, (int32_t)word, Q, w, vt);
+#pragma unroll
+            for (int q = 0; q < R; ++q) h[q] = hn[q];
+        }
+#pragma unroll
+        for (int k = R; k < TM; k += R) {
+            uint32_t hn[R];
+            cell_block_byte<R, SEL>(&a[k], &h[k], hn, (int32_t)word, vt);
+#pragma unroll
+            for (int q = 0; q < R; ++q) h[k + q] = hn[q];
+        }
+        Q = w;    // moved down one lane, lane 63 holding the strand that left the band
+        v = vt;
+        return;
+    }
+    const uint32_t vn = shr1_keep(Q, v);
+    Q = shl1_keep(v, Q);
+    v = vn;
+    bool active = true;
+    if (COLMASK) {
+        const int j = t - lane;
+        active = (j >= 0) && (j < n);
+    }
+    if (active) column_cells8<TM, ROWMASK, SEL>(a, h, valid, word, v);
+}
+
+// Steps t0 .. t0+C-1 of a band whose column symbols are bytes: sym[s / 4]
+// byte s % 4 is the symbol of the column this lane combs at step t0 + s
+// (every lane holds its own C symbols), so only the strand conveyor Q
+// moves between lanes.
+template <int TM, int C, bool ROWMASK, bool COLMASK>
+__device__ __forceinline__ void band_steps8(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
+                                            uint32_t &v, uint32_t &Q, const uint32_t (&sym)[C / 4], int t0, int lane,
+                                            int n)
+{
+#ifndef LCS_WORD_BLOCKS
+#define LCS_WORD_BLOCKS 1
+#endif
+    if constexpr (LCS_WORD_BLOCKS && !ROWMASK && !COLMASK && TM >= 4 && TM <= 16) {
+        // the four steps of a symbol word in one asm block (cell_asm.h)
+#pragma unroll
+        for (int w = 0; w < C / 4; ++w) {
+            uint32_t vw = v, vt;
+            cell_word<TM>(a, h, (int32_t)sym[w], Q, vw, vt);
+            Q = vw;
+            v = vt;
+        }
+        return;
+    }
+#pragma unroll
+    for (int w = 0; w < C / 4; ++w) {
+        const int t = t0 + 4 * w;
+        band_step8<TM, ROWMASK, COLMASK, 0>(a, h, valid, v, Q, sym[w], t, lane, n);
+        band_step8<TM, ROWMASK, COLMASK, 1>(a, h, valid, v, Q, sym[w], t + 1, lane, n);
+        band_step8<TM, ROWMASK, COLMASK, 2>(a, h, valid, v, Q, sym[w], t + 2, lane, n);
+        band_step8<TM, ROWMASK, COLMASK, 3>(a, h, valid, v, Q, sym[w], t + 3, lane, n);
+    }
+}
+
 // Every band runs the same sequence of memory operations: it reads its top
 // edge from ring slot g&1 (tag g) and writes its bottom edge to slot (g+1)&1
 // (tag g+1).  The prep kernel seeds slot 0 with the grid's top edge (tag 0)
@@ -201,8 +298,12 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
 // needed.
 constexpr uint32_t kPadRow16 = 0xffffu, kPadCol16 = 0xfffeu;
 
-template <int TM, int C, bool ROWMASK, bool PK, typename Desc>
-__device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, int t_id)
+// S8: the column symbols come from the byte copies of b (P.sym8) instead of
+// b; every lane loads its own C symbols of the next chunk (C/4 words from
+// the copy that starts its window on a word boundary) and no symbol
+// conveyor runs.
+template <int TM, int C, bool ROWMASK, bool PK, typename Desc, bool S8 = false>
+__device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, int t_id)
 {
     const int m = P.m, n = P.n;
     const int r0 = g * (kWave * TM) + lane * TM;
@@ -253,6 +354,24 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
         }
     };
 
+    // Byte symbols: lane L combs column t - L at step t, the byte at
+    // x0 + t of the padded symbol string (x0 = kSym8Pad + col0 - L >= 65);
+    // copy j = x0 % 4 holds it at byte x0 - j + t, a word boundary for
+    // every chunk (t0 is a multiple of C).  The copies extend kSym8Pad
+    // bytes past column n, beyond the last prefetch (t0 <= n + 64 + C).
+    static_assert(!S8 || (C % 4 == 0 && kWave + 2 * C - 2 < kSym8Pad), "byte-symbol window");
+    [[maybe_unused]] uint32_t sym[C / 4], psym[C / 4];
+    [[maybe_unused]] gsym8 *sym_base = nullptr;
+    if constexpr (S8) {
+        const int x0 = kSym8Pad + P.sym8_col0 - lane;
+        const int j = x0 & 3;
+        sym_base = (gsym8 *)P.sym8 + (size_t)j * (size_t)P.sym8_stride + ((x0 - j) >> 2);
+    }
+    auto load_sym = [&](uint32_t (&d)[C / 4], int t) {
+        gsym8 *p = sym_base + (t >> 2);
+#pragma unroll
+        for (int q = 0; q < C / 4; ++q) d[q] = p[q];
+    };
     // prefetch chunk 0.  Every lane loads from a clamped in-bounds address
     // (no exec masking, no select), so the waits before each chunk cover
     // exactly these loads.
@@ -260,7 +379,9 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
     // Only lanes < C use their words (the conveyors start at lane 0 and
     // shift C times); lanes >= C load the same addresses, so each chunk
     // fetches C words, not 64.
-    int32_t pB = load_b(min(lane & (C - 1), nm1));
+    int32_t pB = 0;
+    if constexpr (S8) load_sym(sym, 0);
+    else pB = load_b(min(lane & (C - 1), nm1));
     unsigned long long pE = ld_relaxed(Ein + min(lane & (C - 1), nm1));
     // Publishes the C bottom-edge strands caught in O during the chunk that
     // started at step t: O lanes [64-C, 64) hold columns t-64 .. t-64+C-1.
@@ -295,6 +416,10 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
         const int c = t + (lane & (C - 1));   // lanes >= C repeat lanes < C
         const bool need = lane < C && (decltype(inside)::value || c < n);
         unsigned long long e = pE;
+        // keeps the tag test (and so the wait for the prefetched word) after
+        // the chunk's cell blocks: the scheduler may otherwise hoist the
+        // compare next to the load and wait there for the whole round trip
+        asm volatile("" : "+v"(e));
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
             // The band above is not that far yet: poll.  Every round also
@@ -333,19 +458,27 @@ __device__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, in
     using Full = std::integral_constant<bool, false>;
     auto chunk = [&](auto colmask, int k) -> bool {
         const int t0 = k * C;
+        if constexpr (S8) load_sym(psym, t0 + C);
         if constexpr (decltype(colmask)::value) {
             const int c = min(t0 + C + (lane & (C - 1)), nm1);
-            pB = load_b(c);
+            if constexpr (!S8) pB = load_b(c);
             pE = ld_relaxed(Ein + c);
         } else {   // an interior chunk: the next chunk's columns are inside [0, n)
             const uint32_t c = (uint32_t)(t0 + C + (lane & (C - 1)));
-            pB = load_b(c);
+            if constexpr (!S8) pB = load_b(c);
             pE = ld_relaxed(Ein + c);
         }
         publish(Osave, t0 - C, std::integral_constant<bool, !decltype(colmask)::value>{});
-        band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, Q, IB, t0, lane, n);
+        if constexpr (S8)
+            band_steps8<TM, C, ROWMASK, decltype(colmask)::value>(a, h, valid, v, Q, sym, t0, lane, n);
+        else
+            band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, Q, IB, t0, lane, n);
         Osave = Q;
         if (!decltype(colmask)::value || k + 1 < nchunks) {   // an interior chunk is never the last
+            if constexpr (S8) {
+#pragma unroll
+                for (int q = 0; q < C / 4; ++q) sym[q] = psym[q];
+            }
             IB = pB;
             if (!take_inputs(t0 + C, Q, std::integral_constant<bool, !decltype(colmask)::value>{})) return false;
         }
@@ -396,8 +529,15 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         const PairDesc P = A.pairs[task.x];
         const int g = task.y;
         const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
-        const bool ok = rowmask ? band_pass<TM, C, true, false>(A, P, g, lane, (int)t)
-                                : band_pass<TM, C, false, false>(A, P, g, lane, (int)t);
+        // byte symbols when the pair has them and all of b fits in a byte
+        const bool s8 = P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
+        bool ok;
+        if (s8)
+            ok = rowmask ? band_pass<TM, C, true, false, PairDesc, true>(A, P, g, lane, (int)t)
+                         : band_pass<TM, C, false, false, PairDesc, true>(A, P, g, lane, (int)t);
+        else
+            ok = rowmask ? band_pass<TM, C, true, false>(A, P, g, lane, (int)t)
+                         : band_pass<TM, C, false, false>(A, P, g, lane, (int)t);
         if (!ok) break;
     }
 }
@@ -760,6 +900,32 @@ __global__ void lcsk_zero_kernel(int32_t *x, int32_t n)
     if (i < n) x[i] = 0;
 }
 
+// Byte symbols of b (the layout lcsk_sym8_stride describes): one thread per
+// word of the four copies; copy 0 (which covers every column) raises *bad
+// for a symbol outside [0, 255].
+__global__ void comb_sym8_kernel(const int32_t *b, int32_t n, uint32_t *out, int32_t stride, int32_t *bad)
+{
+    const long long total = 4LL * stride;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += step) {
+        const int j = (int)(q / stride);
+        const long long x = 4 * (q - (long long)j * stride) + j - kSym8Pad;   // column of byte 0
+        uint32_t word = 0;
+        bool out_of_range = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long c = x + i;
+            if (c >= 0 && c < n) {
+                const int32_t sym = b[c];
+                out_of_range |= (uint32_t)sym > 255u;
+                word |= ((uint32_t)sym & 0xffu) << (8 * i);
+            }
+        }
+        out[q] = word;
+        if (out_of_range && j == 0) atomicOr(bad, 1);
+    }
+}
+
 // Range check of the packed chain's symbols (blockIdx.y = string).
 __global__ void check16_kernel(const SymbolRange *ranges, int32_t *bad)
 {
@@ -884,6 +1050,16 @@ int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs
         if (rc) return rc;
     }
     hipLaunchKernelGGL(comb_extract_kernel, dim3(fill_blocks, npairs), dim3(256), 0, stream, d_pairs);
+    return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int lcsk_launch_sym8(const int32_t *b, int32_t n, uint32_t *out, int32_t stride, int32_t *bad, hipStream_t stream)
+{
+    if (n <= 0 || stride < lcsk_sym8_stride(n)) return -22;
+    hipLaunchKernelGGL(lcsk_zero_kernel, dim3(1), dim3(64), 0, stream, bad, 1);
+    const long long words = 4LL * stride;
+    const unsigned blocks = (unsigned)std::min<long long>((words + 255) / 256, 8192);
+    hipLaunchKernelGGL(comb_sym8_kernel, dim3(blocks), dim3(256), 0, stream, b, n, out, stride, bad);
     return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 

@@ -339,6 +339,36 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
     return s;
 }
 
+// Byte symbols for the register chain (lcsk_launch_sym8): the pairs, all
+// reading columns of the one string d_b (pair p from column col0[p]), get
+// the byte copies of d_b; a chain band then takes each lane's symbols from
+// them when every symbol of d_b lies in [0, 255] (which the conversion
+// kernel finds out on the device) and from d_b otherwise.  A b longer than
+// kSym8MaxN, or a context whose budget cannot hold the 4(n + 256) bytes,
+// keeps the 32-bit symbols.
+#ifndef LCS_SYM8
+#define LCS_SYM8 1
+#endif
+constexpr int32_t kSym8MaxN = 1 << 30;
+int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc> &pairs,
+               const std::vector<int32_t> &col0, hipStream_t stream)
+{
+    if (!LCS_SYM8 || n <= 0 || n > kSym8MaxN) return LCS_OK;
+    const int32_t stride = lcsk_sym8_stride(n);
+    const size_t bytes = (size_t)4 * 4 * (size_t)stride + 64;
+    if (ctx->sym8.reserve(ctx, bytes) != LCS_OK) return LCS_OK;
+    uint32_t *copies = (uint32_t *)ctx->sym8.ptr;
+    int32_t *bad = (int32_t *)(copies + (size_t)4 * stride);
+    if (lcsk_launch_sym8(d_b, n, copies, stride, bad, stream)) return LCS_EIO;
+    for (size_t p = 0; p < pairs.size(); ++p) {
+        pairs[p].sym8 = copies;
+        pairs[p].sym8_bad = bad;
+        pairs[p].sym8_col0 = col0[p];
+        pairs[p].sym8_stride = stride;
+    }
+    return LCS_OK;
+}
+
 // One pair cut into `strips` column strips: comb the strips in one chain
 // launch, turn each into its permutation, merge them pairwise with the
 // steady ant (b-split, k = m), and read the strands off the product.
@@ -373,6 +403,11 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
         off[(size_t)s] = acc;
         size[(size_t)s] = (int64_t)m + P.n;
         acc += size[(size_t)s];
+    }
+    if (group_w == 1) {
+        std::vector<int32_t> col0((size_t)S);
+        for (int s = 0; s < S; ++s) col0[(size_t)s] = s * W;
+        if ((rc = stage_sym8(ctx, d_b, n, pairs, col0, stream))) return rc;
     }
     if ((rc = enqueue_pairs(ctx, pairs, tm, wps, stream, group_w))) return rc;
     // enqueue_pairs staged the descriptors into ctx->descs; reuse them for the permutations
@@ -497,6 +532,10 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
     P.v_out = d_v;
     P.m = m;
     P.n = n;
+    if (group_w == 1) {
+        const int rc = stage_sym8(ctx, d_b, n, pairs, std::vector<int32_t>(1, 0), stream);
+        if (rc) return rc;
+    }
     return enqueue_pairs(ctx, pairs, tm, wps, stream, group_w);
 }
 
@@ -582,6 +621,7 @@ int lcs_ctx_destroy(lcs_ctx *ctx)
     ctx->ring.release(ctx);
     ctx->descs.release(ctx);
     ctx->strip_ws.release(ctx);
+    ctx->sym8.release(ctx);
     ctx->staging.destroy();
     if (ctx->ws_done) (void)hipEventDestroy(ctx->ws_done);
     ctx->in_a.release(ctx);

@@ -83,6 +83,7 @@ struct lcs_ctx {
     double last_kernel_ms = 0.0;
     bool pending_timing = false;
     lcs::DevBuf ring, in_a, in_b, out_h, out_v, descs, strip_ws, perm;
+    lcs::DevBuf sym8;                    // byte copies of b and their range flag (single-pair chains)
     // The shared workspace (ring, descriptors, ctrl words) is used by one
     // launch at a time: a call on another stream first waits for ws_done,
     // recorded after the last call's kernels on ws_stream.
