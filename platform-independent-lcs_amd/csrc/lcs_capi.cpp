@@ -345,11 +345,19 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 // them when every symbol of d_b lies in [0, 255] (which the conversion
 // kernel finds out on the device) and from d_b otherwise.  A b longer than
 // kSym8MaxN, or a context whose budget cannot hold the 4(n + 256) bytes,
-// keeps the 32-bit symbols.
+// keeps the 32-bit symbols.  The staging is two small launches (about 8 us
+// of device time at n = 16384); a default-variant pair below kSym8MinCells
+// cells, whose comb that would not repay, skips it, and an explicit
+// LCS_VARIANT_CHAIN takes it at any size.
 #ifndef LCS_SYM8
 #define LCS_SYM8 1
 #endif
 constexpr int32_t kSym8MaxN = 1 << 30;
+constexpr double kSym8MinCells = 1073741824.0;   // 2^30
+bool use_sym8(int variant, int32_t m, int32_t n)
+{
+    return variant == LCS_VARIANT_CHAIN || (double)m * (double)n >= kSym8MinCells;
+}
 int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc> &pairs,
                const std::vector<int32_t> &col0, hipStream_t stream)
 {
@@ -373,7 +381,7 @@ int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc
 // launch, turn each into its permutation, merge them pairwise with the
 // steady ant (b-split, k = m), and read the strands off the product.
 int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n, int strips, int tm,
-                   int wps, int group_w, int32_t *d_h, int32_t *d_v, hipStream_t stream)
+                   int wps, int group_w, bool sym8, int32_t *d_h, int32_t *d_v, hipStream_t stream)
 {
     const int32_t W = (n + strips - 1) / strips;
     const int S = (n + W - 1) / W;
@@ -404,7 +412,7 @@ int enqueue_strips(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d
         size[(size_t)s] = (int64_t)m + P.n;
         acc += size[(size_t)s];
     }
-    if (group_w == 1) {
+    if (group_w == 1 && sym8) {
         std::vector<int32_t> col0((size_t)S);
         for (int s = 0; s < S; ++s) col0[(size_t)s] = s * W;
         if ((rc = stage_sym8(ctx, d_b, n, pairs, col0, stream))) return rc;
@@ -520,7 +528,8 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
     if (strips > 1 && (d_h_in || d_v_in)) return LCS_EINVAL;  // fixups use explicit strands: one strip
     if (strips > 1 && m > kMaxCoreK) return LCS_ERANGE;
     if (strips > n) strips = n;
-    if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, group_w, d_h, d_v, stream);
+    const bool sym8 = use_sym8(sched ? sched->variant : LCS_VARIANT_AUTO, m, n);
+    if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, group_w, sym8, d_h, d_v, stream);
     std::vector<PairDesc> pairs(1);
     PairDesc &P = pairs[0];
     P = PairDesc{};
@@ -532,7 +541,7 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
     P.v_out = d_v;
     P.m = m;
     P.n = n;
-    if (group_w == 1) {
+    if (group_w == 1 && sym8) {
         const int rc = stage_sym8(ctx, d_b, n, pairs, std::vector<int32_t>(1, 0), stream);
         if (rc) return rc;
     }
