@@ -530,7 +530,9 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         const int g = task.y;
         const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
         // byte symbols when the pair has them and all of b fits in a byte
-        const bool s8 = P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
+        // (not at one row per lane, where a step is a single cell and the
+        // symbol conveyor costs less than the hazard wait it fills)
+        const bool s8 = TM >= 2 && P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
         bool ok;
         if (s8)
             ok = rowmask ? band_pass<TM, C, true, false, PairDesc, true>(A, P, g, lane, (int)t)
