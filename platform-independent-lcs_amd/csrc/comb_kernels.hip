@@ -179,7 +179,20 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
             active = (j >= 0) && (j < n);
         }
         if (active) {
-            if constexpr (PK) {
+            if constexpr (PK && !COLMASK && TM >= 4) {
+                // software-pipelined rows (cell_asm.h): no packed result is
+                // read by the next issue, so no s_nop between the rows
+                constexpr int R = TM >= 16 ? 16 : TM;
+#pragma unroll
+                for (int k = 0; k < TM; k += R) {
+                    uint32_t ua[R], hn[R];
+#pragma unroll
+                    for (int q = 0; q < R; ++q) ua[q] = (uint32_t)a[k + q];
+                    cell16_block<R>(ua, &h[k], hn, (uint32_t)bq, v);
+#pragma unroll
+                    for (int q = 0; q < R; ++q) h[k + q] = hn[q];
+                }
+            } else if constexpr (PK) {
 #pragma unroll
                 for (int k = 0; k < TM; ++k) cell16((uint32_t)a[k], (uint32_t)bq, h[k], v);
             } else {

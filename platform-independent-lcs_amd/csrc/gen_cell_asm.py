@@ -102,6 +102,42 @@ def word_block(R: int) -> str:
             f"        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n}}\n")
 
 
+def packed_block(R: int) -> str:
+    """One step of the packed (two pairs in 16-bit halves) chain for R >= 3
+    rows.  Per row k: X e = a ^ b, S mk = sat(1 - e), M hx = sat(mk * 0xffff
+    + h), N hn = min(hx, v), B v = h ^ v ^ hn (hn is the row's new h).  A
+    packed (VOP3P) result read by the next VALU issue costs a wait state on
+    gfx950, so the rows are software-pipelined: row k's N and B run with row
+    k+1's M, row k+3's X and row k+2's S between them, in the order
+    N(k) M(k+1) B(k) X(k+3) S(k+2), so every packed result is read two or
+    more issues later and only the last row's N -> B pair needs an s_nop.
+    Four rotating temporaries carry e -> mk -> hx of the rows in flight."""
+    assert R >= 3
+    def X(k): return f'"v_xor_b32_e32 %[r{k % 4}], %[a{k}], %[b]\\n\\t"'
+    def S(k): return f'"v_pk_sub_u16 %[r{k % 4}], 1, %[r{k % 4}] op_sel_hi:[0,1] clamp\\n\\t"'
+    def M(k): return f'"v_pk_mad_u16 %[r{k % 4}], %[r{k % 4}], -1, %[h{k}] clamp\\n\\t"'
+    def N(k): return f'"v_pk_min_u16 %[n{k}], %[r{k % 4}], %[v]\\n\\t"'
+    def B(k): return f'"v_bitop3_b32 %[v], %[h{k}], %[v], %[n{k}] bitop3:0x96\\n\\t"'
+    ins = [X(0), S(0), X(1), M(0), S(1), X(2)]
+    for k in range(R):
+        ins.append(N(k))
+        if k + 1 < R:
+            ins.append(M(k + 1))
+        else:
+            ins.append('"s_nop 0\\n\\t"')
+        ins.append(B(k))
+        if k + 3 < R:
+            ins.append(X(k + 3))
+        if k + 2 < R:
+            ins.append(S(k + 2))
+    outs = ['[v] "+v"(v)'] + [f'[n{k}] "=&v"(hn[{k}])' for k in range(R)] + [f'[r{q}] "=&v"(r{q})' for q in range(4)]
+    inp = ['[b] "v"(sb)'] + [f'[a{k}] "v"(a[{k}])' for k in range(R)] + [f'[h{k}] "v"(h[{k}])' for k in range(R)]
+    body = "\n        ".join(ins)
+    return (f"__device__ __forceinline__ void cell16_block{R}(const uint32_t *a, const uint32_t *h, uint32_t *hn,"
+            f" uint32_t sb, uint32_t &v)\n{{\n    uint32_t r0, r1, r2, r3;\n    asm volatile(\n"
+            f"        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n}}\n")
+
+
 if __name__ == "__main__":
     print("// Generated by gen_cell_asm.py -- hand-scheduled cell blocks (see there).")
     print("#pragma once\n#include <stdint.h>\n")
@@ -125,6 +161,15 @@ if __name__ == "__main__":
             print(step_block(R, q))
     for R in (4, 8, 16):
         print(word_block(R))
+    for R in (4, 8, 16):
+        print(packed_block(R))
+    print("// cell16_block{R} by template argument")
+    print("template <int R>\n__device__ __forceinline__ void cell16_block(const uint32_t *a, const uint32_t *h,"
+          " uint32_t *hn, uint32_t sb, uint32_t &v)\n{")
+    print("    if constexpr (R == 4) cell16_block4(a, h, hn, sb, v);")
+    print("    else if constexpr (R == 8) cell16_block8(a, h, hn, sb, v);")
+    print("    else if constexpr (R == 16) cell16_block16(a, h, hn, sb, v);")
+    print('    else static_assert(R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
     print("// cell_word{R} by template argument")
     print("template <int R>\n__device__ __forceinline__ void cell_word(const int32_t *a, uint32_t *h, int32_t sb,"
           " uint32_t q, uint32_t &w, uint32_t &t)\n{")
