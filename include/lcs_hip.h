@@ -61,8 +61,8 @@ typedef struct lcs_schedule {
 #define LCS_VARIANT_AUTO 0
 #define LCS_VARIANT_CHAIN 1      /* wave64 band chain, registers only         */
 /* The register chain reads b as bytes when every symbol of b lies in
- * [0, 255] (checked on the device per call) and tile_m >= 2: each lane then
- * holds its own column symbols and none move between lanes.
+ * [0, 255] (checked on the device per call): each lane then holds its own
+ * column symbols and none move between lanes.
  * LCS_VARIANT_AUTO stages the bytes for pairs of 2^30 cells or more,
  * LCS_VARIANT_CHAIN at any size; the results are the same either way. */
 #define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */

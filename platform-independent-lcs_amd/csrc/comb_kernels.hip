@@ -277,7 +277,7 @@ __device__ __forceinline__ void band_steps8(const int32_t (&a)[TM], uint32_t (&h
 #ifndef LCS_WORD_BLOCKS
 #define LCS_WORD_BLOCKS 1
 #endif
-    if constexpr (LCS_WORD_BLOCKS && !ROWMASK && !COLMASK && TM >= 4 && TM <= 16) {
+    if constexpr (LCS_WORD_BLOCKS && !ROWMASK && !COLMASK && TM <= 16) {
         // the four steps of a symbol word in one asm block (cell_asm.h)
 #pragma unroll
         for (int w = 0; w < C / 4; ++w) {
@@ -430,9 +430,10 @@ __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int
         const bool need = lane < C && (decltype(inside)::value || c < n);
         unsigned long long e = pE;
         // keeps the tag test (and so the wait for the prefetched word) after
-        // the chunk's cell blocks: the scheduler may otherwise hoist the
-        // compare next to the load and wait there for the whole round trip
-        asm volatile("" : "+v"(e));
+        // the chunk's last step: I still holds the conveyor the chunk left,
+        // which that step wrote, so the compare cannot be hoisted next to
+        // the load to wait there for the whole round trip
+        asm volatile("" : "+v"(e) : "v"(I));
         bool ok = !need || (uint32_t)(e >> 32) == tag_in;
         if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
             // The band above is not that far yet: poll.  Every round also
@@ -543,9 +544,7 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         const int g = task.y;
         const bool rowmask = (long long)(g + 1) * (kWave * TM) > (long long)P.m;
         // byte symbols when the pair has them and all of b fits in a byte
-        // (not at one row per lane, where a step is a single cell and the
-        // symbol conveyor costs less than the hazard wait it fills)
-        const bool s8 = TM >= 2 && P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
+        const bool s8 = P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
         bool ok;
         if (s8)
             ok = rowmask ? band_pass<TM, C, true, false, PairDesc, true>(A, P, g, lane, (int)t)

@@ -82,7 +82,7 @@ def word_block(R: int) -> str:
     for sel in range(4):
         conv, strand, old = regs[sel]
         src_h, dst_h = ("h", "x") if sel % 2 == 0 else ("x", "h")
-        ins += [cmp(sel, "s0", "a0"), cmp(sel, "s1", "a1"),
+        ins += [cmp(sel, "s0", "a0")] + ([cmp(sel, "s1", "a1")] if R > 1 else []) + [
                 f'"v_mov_b32_e32 %[{strand}], %[{conv}]\\n\\t"',
                 f'"v_mov_b32_dpp %[{strand}], %[{old}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
                 f'"v_mov_b32_dpp %[{old}], %[{conv}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
@@ -159,7 +159,7 @@ if __name__ == "__main__":
     for R in (4, 8, 16):
         for q in range(4):
             print(step_block(R, q))
-    for R in (4, 8, 16):
+    for R in (1, 2, 4, 8, 16):
         print(word_block(R))
     for R in (4, 8, 16):
         print(packed_block(R))
@@ -173,10 +173,12 @@ if __name__ == "__main__":
     print("// cell_word{R} by template argument")
     print("template <int R>\n__device__ __forceinline__ void cell_word(const int32_t *a, uint32_t *h, int32_t sb,"
           " uint32_t q, uint32_t &w, uint32_t &t)\n{")
-    print("    if constexpr (R == 4) cell_word4(a, h, sb, q, w, t);")
+    print("    if constexpr (R == 1) cell_word1(a, h, sb, q, w, t);")
+    print("    else if constexpr (R == 2) cell_word2(a, h, sb, q, w, t);")
+    print("    else if constexpr (R == 4) cell_word4(a, h, sb, q, w, t);")
     print("    else if constexpr (R == 8) cell_word8(a, h, sb, q, w, t);")
     print("    else if constexpr (R == 16) cell_word16(a, h, sb, q, w, t);")
-    print('    else static_assert(R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
+    print('    else static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
     print("// cell_step{R}_byte{SEL} by template arguments")
     print("template <int R, int SEL>\n__device__ __forceinline__ void cell_step_byte(const int32_t *a, const uint32_t *h,"
           " uint32_t *hn, int32_t sb, uint32_t q, uint32_t &w, uint32_t &t)\n{")

@@ -348,18 +348,15 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 // keeps the 32-bit symbols.  The staging is two small launches (about 8 us
 // of device time at n = 16384); a default-variant pair below kSym8MinCells
 // cells, whose comb that would not repay, skips it, and an explicit
-// LCS_VARIANT_CHAIN takes it at any size.  One row per lane keeps 32-bit
-// symbols (m = 2^14, n = 2^22, one strip: 94.8 ms against 110.8 with bytes,
-// where two rows per lane gain 157.1 -> 151.0 and eight 46.1 -> 44.7 at
-// m = n = 2^19).
+// LCS_VARIANT_CHAIN takes it at any size.
 #ifndef LCS_SYM8
 #define LCS_SYM8 1
 #endif
 constexpr int32_t kSym8MaxN = 1 << 30;
 constexpr double kSym8MinCells = 1073741824.0;   // 2^30
-bool use_sym8(int variant, int32_t m, int32_t n, int tm)
+bool use_sym8(int variant, int32_t m, int32_t n)
 {
-    return tm >= 2 && (variant == LCS_VARIANT_CHAIN || (double)m * (double)n >= kSym8MinCells);
+    return variant == LCS_VARIANT_CHAIN || (double)m * (double)n >= kSym8MinCells;
 }
 int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc> &pairs,
                const std::vector<int32_t> &col0, hipStream_t stream)
@@ -531,7 +528,7 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
     if (strips > 1 && (d_h_in || d_v_in)) return LCS_EINVAL;  // fixups use explicit strands: one strip
     if (strips > 1 && m > kMaxCoreK) return LCS_ERANGE;
     if (strips > n) strips = n;
-    const bool sym8 = use_sym8(sched ? sched->variant : LCS_VARIANT_AUTO, m, n, tm);
+    const bool sym8 = use_sym8(sched ? sched->variant : LCS_VARIANT_AUTO, m, n);
     if (strips > 1) return enqueue_strips(ctx, d_a, m, d_b, n, strips, tm, wps, group_w, sym8, d_h, d_v, stream);
     std::vector<PairDesc> pairs(1);
     PairDesc &P = pairs[0];
