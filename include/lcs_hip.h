@@ -62,9 +62,9 @@ typedef struct lcs_schedule {
 #define LCS_VARIANT_CHAIN 1      /* wave64 band chain, registers only         */
 /* The register chain reads b as bytes when every symbol of b lies in
  * [0, 255] (checked on the device per call): each lane then holds its own
- * column symbols and none move between lanes.
- * LCS_VARIANT_AUTO stages the bytes for pairs of 2^30 cells or more,
- * LCS_VARIANT_CHAIN at any size; the results are the same either way. */
+ * column symbols and none move between lanes.  The bytes are staged (two
+ * small launches) for pairs of 2^30 cells or more, and at any size with
+ * LCS_VARIANT_CHAIN_BYTES; the results are the same either way. */
 #define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */
 /* Batches only: pairs of a batch with this variant are combed two per wave
  * with 16-bit strands (pair X in the low, pair Y in the high half of every
@@ -73,6 +73,9 @@ typedef struct lcs_schedule {
  * (and an odd one out) take the register chain.  Same bits as every other
  * variant.  Elsewhere it means LCS_VARIANT_CHAIN. */
 #define LCS_VARIANT_PACKED16 3
+/* LCS_VARIANT_CHAIN with the byte symbols staged whatever the pair's size
+ * (single pairs; in a batch it means LCS_VARIANT_CHAIN). */
+#define LCS_VARIANT_CHAIN_BYTES 4
 
 /* ---- context ------------------------------------------------------------
  * Replaces the registry-owned sycl::queue (src/algorithm_registry.cpp:6-41,

@@ -125,7 +125,8 @@ int group_width(int variant)
 {
     switch (variant) {
     case LCS_VARIANT_AUTO:
-    case LCS_VARIANT_CHAIN: return 1;
+    case LCS_VARIANT_CHAIN:
+    case LCS_VARIANT_CHAIN_BYTES: return 1;
     case LCS_VARIANT_CHAIN_LDS: return 4;
     case LCS_VARIANT_PACKED16: return 1;   // one pair per wave outside batches
     default: return 0;
@@ -346,9 +347,9 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 // kernel finds out on the device) and from d_b otherwise.  A b longer than
 // kSym8MaxN, or a context whose budget cannot hold the 4(n + 256) bytes,
 // keeps the 32-bit symbols.  The staging is two small launches (about 8 us
-// of device time at n = 16384); a default-variant pair below kSym8MinCells
-// cells, whose comb that would not repay, skips it, and an explicit
-// LCS_VARIANT_CHAIN takes it at any size.
+// of device time at n = 16384); a pair below kSym8MinCells cells, whose
+// comb that would not repay, skips it unless its schedule asks for
+// LCS_VARIANT_CHAIN_BYTES.
 #ifndef LCS_SYM8
 #define LCS_SYM8 1
 #endif
@@ -356,7 +357,7 @@ constexpr int32_t kSym8MaxN = 1 << 30;
 constexpr double kSym8MinCells = 1073741824.0;   // 2^30
 bool use_sym8(int variant, int32_t m, int32_t n)
 {
-    return variant == LCS_VARIANT_CHAIN || (double)m * (double)n >= kSym8MinCells;
+    return variant == LCS_VARIANT_CHAIN_BYTES || (double)m * (double)n >= kSym8MinCells;
 }
 int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc> &pairs,
                const std::vector<int32_t> &col0, hipStream_t stream)

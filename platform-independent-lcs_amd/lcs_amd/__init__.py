@@ -27,7 +27,7 @@ ERRORS = {
     -34: "ERANGE", -110: "ETIMEDOUT", -125: "ECANCELED",
 }
 
-VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS, VARIANT_PACKED16 = 0, 1, 2, 3
+VARIANT_AUTO, VARIANT_CHAIN, VARIANT_CHAIN_LDS, VARIANT_PACKED16, VARIANT_CHAIN_BYTES = 0, 1, 2, 3, 4
 TUNE_SINGLE, TUNE_BATCH = 0, 1
 SCHED_FROM = {0: "default", 1: "table", 2: "model", 3: "autotune", 4: "caller"}
 SEMI_STRING_SUBSTRING, SEMI_SUBSTRING_STRING, SEMI_PREFIX_SUFFIX, SEMI_SUFFIX_PREFIX, SEMI_DOMINANCE = 0, 1, 2, 3, 4

@@ -6,7 +6,7 @@ import lcs_amd
 import oracle
 
 pytestmark = pytest.mark.gpu
-CHAIN = lcs_amd.VARIANT_CHAIN   # the register chain, byte symbols at any size when b allows them
+BYTES = lcs_amd.VARIANT_CHAIN_BYTES   # the register chain, byte symbols staged at any size when b allows them
 
 
 def _check(ctx, a, b, sched=None):
@@ -31,7 +31,7 @@ def test_edge_shapes(ctx, m, n):
     b = rng.integers(0, 3, n).astype(np.int32)
     for tm in (1, 4, 16):
         _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=CHAIN))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=BYTES))
 
 
 def test_interior_chunk_bounds(ctx):
@@ -49,7 +49,7 @@ def test_interior_chunk_bounds(ctx):
         b = rng.integers(0, 3, n).astype(np.int32)
         for tm in (4, 16):
             _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
-            _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=CHAIN))
+            _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=BYTES))
         _check(ctx, a, b, lcs_amd.schedule(tile_m=4, variant=LDS))
 
 
@@ -60,7 +60,7 @@ def test_tile_heights(ctx, tm):
         a = rng.integers(0, 4, m).astype(np.int32)
         b = rng.integers(0, 4, n).astype(np.int32)
         _check(ctx, a, b, lcs_amd.schedule(tile_m=tm))
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=CHAIN))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=BYTES))
 
 
 def test_uniform4096(ctx):
@@ -89,7 +89,7 @@ def test_large_alphabet_and_negative_symbols(ctx):
 def test_byte_symbols_and_their_range_check(ctx):
     """The register chain reads byte copies of b when every symbol of b lies
     in [0, 255] (a device-side check per call) and b itself otherwise (the
-    explicit chain variant stages the copies at any size).  Row symbols
+    byte-chain variant stages the copies at any size).  Row symbols
     congruent to a column symbol mod 256 must not match it, a single symbol
     outside the byte range anywhere in b must switch the call to the 32-bit
     symbols, and the next call with byte symbols must switch back; strips
@@ -100,20 +100,20 @@ def test_byte_symbols_and_their_range_check(ctx):
     a = rng.integers(0, 256, m).astype(np.int32)
     a[::3] += 256                                   # never equal to a byte symbol
     for tm in (1, 2, 4, 8, 16, 32):
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=CHAIN))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=tm, variant=BYTES))
     for last in (256, -1, 1 << 20):
         bb = b.copy()
         bb[-1] = last
         aa = a.copy()
         aa[::5] = last                              # matches only in 32 bits
-        _check(ctx, aa, bb, lcs_amd.schedule(tile_m=16, variant=CHAIN))
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=16, variant=CHAIN))
+        _check(ctx, aa, bb, lcs_amd.schedule(tile_m=16, variant=BYTES))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=16, variant=BYTES))
     bb = b.copy()
     bb[0] = 300
-    _check(ctx, a, bb, lcs_amd.schedule(tile_m=4, strips=3, variant=CHAIN))
+    _check(ctx, a, bb, lcs_amd.schedule(tile_m=4, strips=3, variant=BYTES))
     for strips in (2, 3, 7):                        # strip offsets of every residue mod 4
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=4, strips=strips, variant=CHAIN))
-        _check(ctx, a, b, lcs_amd.schedule(tile_m=16, strips=strips, variant=CHAIN))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=4, strips=strips, variant=BYTES))
+        _check(ctx, a, b, lcs_amd.schedule(tile_m=16, strips=strips, variant=BYTES))
 
 
 def test_all_match_and_no_match(ctx):
