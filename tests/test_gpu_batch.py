@@ -41,8 +41,11 @@ def test_batch_equals_single_calls(ctx):
 
 
 def test_batch_mixed_variants(ctx):
+    # register chain, LDS groups, and the byte-chain variant (in a batch the
+    # register chain with 32-bit symbols)
     pairs = lcs_amd.workload_batch(40, seed=21, lo=1, hi=5000)
-    scheds = [lcs_amd.schedule(tile_m=[1, 2, 4, 8][i % 4], variant=1 + (i % 2)) for i in range(len(pairs))]
+    variants = [lcs_amd.VARIANT_CHAIN, lcs_amd.VARIANT_CHAIN_LDS, lcs_amd.VARIANT_CHAIN_BYTES]
+    scheds = [lcs_amd.schedule(tile_m=[1, 2, 4, 8][i % 4], variant=variants[i % 3]) for i in range(len(pairs))]
     hs, vs = ctx.batch_comb([p.a for p in pairs], [p.b for p in pairs], scheds=scheds)
     for p, h, v in zip(pairs, hs, vs):
         ho, vo = oracle.comb(p.a, p.b)
