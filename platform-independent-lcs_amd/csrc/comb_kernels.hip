@@ -227,9 +227,11 @@ __device__ __forceinline__ void column_cells8(const int32_t (&a)[TM], uint32_t (
 
 // Step t of a band with byte symbols: the strand moves as in band_steps;
 // the column symbol is byte SEL of `word`, which the lane holds itself.
-// Unmasked steps run as one asm block per step (cell_asm.h: the strand
-// hand-off sits behind the step's first two compares, so the DPP read of v
-// needs no s_nop after the previous step's last cell wrote it).
+// An unmasked step of four or more rows runs as one asm block (cell_asm.h:
+// the strand hand-off sits behind the step's first two compares, so the DPP
+// read of v needs no s_nop after the previous step's last cell wrote it);
+// band_steps8 uses this for 32-row tiles, and for every tile height when
+// its four-step blocks are compiled out.
 template <int TM, bool ROWMASK, bool COLMASK, int SEL>
 __device__ __forceinline__ void band_step8(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
                                            uint32_t &v, uint32_t &Q, uint32_t word, int t, int lane, int n)
@@ -274,6 +276,9 @@ __device__ __forceinline__ void band_steps8(const int32_t (&a)[TM], uint32_t (&h
                                             uint32_t &v, uint32_t &Q, const uint32_t (&sym)[C / 4], int t0, int lane,
                                             int n)
 {
+// LCS_WORD_BLOCKS=0 runs one asm block per step instead of per symbol word
+// (the compiler then puts an s_nop 0 before every step: m = n = 2^20 took
+// 133.6 ms per comb against 132.1)
 #ifndef LCS_WORD_BLOCKS
 #define LCS_WORD_BLOCKS 1
 #endif
