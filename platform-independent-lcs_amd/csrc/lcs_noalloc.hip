@@ -27,6 +27,7 @@ namespace {
 constexpr int32_t kNoAllocMaxN = 1 << 21;
 constexpr int32_t kNoAllocMaxTasks = 1 << 16;
 constexpr size_t kCtrlOff = 0, kDescOff = 256, kTaskOff = 512;
+static_assert(sizeof(PairDesc) <= kTaskOff - kDescOff, "the pair descriptor must fit its scratch slot");
 constexpr size_t kRingOff = kTaskOff + sizeof(int2) * (size_t)kNoAllocMaxTasks;
 constexpr size_t kScratchBytes = kRingOff + 8 * (2 * (size_t)kNoAllocMaxN + 64);
 constexpr unsigned long long kTimeoutTicks = 500000000ull;   // 5 s of the 100 MHz device clock
