@@ -282,6 +282,16 @@ __device__ __forceinline__ void band_steps8(const int32_t (&a)[TM], uint32_t (&h
 #ifndef LCS_WORD_BLOCKS
 #define LCS_WORD_BLOCKS 1
 #endif
+#ifndef LCS_CHUNK_BLOCKS
+#define LCS_CHUNK_BLOCKS 1
+#endif
+    if constexpr (LCS_CHUNK_BLOCKS && C == 32 && !ROWMASK && !COLMASK && TM <= 16) {
+        // the whole chunk in one asm block (cell_asm.h)
+        uint32_t qn;
+        cell_chunk32<TM>(a, h, sym, Q, v, qn);
+        Q = qn;
+        return;
+    }
     if constexpr (LCS_WORD_BLOCKS && !ROWMASK && !COLMASK && TM <= 16) {
         // the four steps of a symbol word in one asm block (cell_asm.h)
 #pragma unroll

@@ -138,6 +138,44 @@ def packed_block(R: int) -> str:
             f"        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n}}\n")
 
 
+def chunk_block(R: int, W: int) -> str:
+    """4W steps (W symbol words) of the byte-symbol band loop for R rows as
+    one block: the compiler's own wait state before a block that reads the
+    strand then comes once per 4W steps.  Step k moves the strand into the
+    one of t, u, w that holds neither the strand nor the conveyor (the
+    conveyor's copy goes there first), so the roles rotate with period three;
+    the rows' h strands alternate between h and a scratch set (4W is even,
+    so they end in h).  Returns the strand in `vt` and the conveyor in `qo`."""
+    def cmp(sel, word, s, a):
+        return f'"v_cmp_eq_u32_sdwa %[{s}], %[{a}], %[b{word}] src0_sel:DWORD src1_sel:BYTE_{sel}\\n\\t"'
+    ins = []
+    strand, conv = "w", "q"           # before step 0: strand in w (the input v), conveyor in q
+    for k in range(4 * W):
+        word, sel = divmod(k, 4)
+        dest = "t" if k == 0 else ({"t", "u", "w"} - {strand, conv}).pop()
+        src_h, dst_h = ("h", "x") if k % 2 == 0 else ("x", "h")
+        ins += [cmp(sel, word, "s0", "a0")] + ([cmp(sel, word, "s1", "a1")] if R > 1 else []) + [
+            f'"v_mov_b32_e32 %[{dest}], %[{conv}]\\n\\t"',
+            f'"v_mov_b32_dpp %[{dest}], %[{strand}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
+            f'"v_mov_b32_dpp %[{strand}], %[{conv}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
+        strand, conv = dest, strand
+        for r in range(R):
+            sr = f"s{r % 2}"
+            ins.append(f'"v_min_u32_e32 %[{dst_h}{r}], %[{src_h}{r}], %[{strand}]\\n\\t"')
+            ins.append(f'"v_cndmask_b32_e64 %[{dst_h}{r}], %[{dst_h}{r}], %[{strand}], %[{sr}]\\n\\t"')
+            ins.append(f'"v_bitop3_b32 %[{strand}], %[{src_h}{r}], %[{strand}], %[{dst_h}{r}] bitop3:0x96\\n\\t"')
+            if r + 2 < R:
+                ins.append(cmp(sel, word, sr, f"a{r + 2}"))
+    outs = ['[w] "+v"(w)', '[t] "=&v"(t)', '[u] "=&v"(u)'] + [f'[h{r}] "+v"(h[{r}])' for r in range(R)] + \
+           [f'[x{r}] "=&v"(x[{r}])' for r in range(R)] + ['[s0] "=&s"(s0)', '[s1] "=&s"(s1)']
+    inp = [f'[b{i}] "v"(sb[{i}])' for i in range(W)] + ['[q] "v"(q)'] + [f'[a{r}] "v"(a[{r}])' for r in range(R)]
+    body = "\n        ".join(ins)
+    return (f"__device__ __forceinline__ void cell_chunk{R}x{W}(const int32_t *a, uint32_t *h, const uint32_t *sb,"
+            f" uint32_t q, uint32_t &vt, uint32_t &qo)\n{{\n    uint64_t s0, s1;\n    uint32_t w = vt, t, u, x[{R}];\n"
+            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n"
+            f"    vt = {strand};\n    qo = {conv};\n}}\n")
+
+
 if __name__ == "__main__":
     print("// Generated by gen_cell_asm.py -- hand-scheduled cell blocks (see there).")
     print("#pragma once\n#include <stdint.h>\n")
@@ -178,6 +216,15 @@ if __name__ == "__main__":
     print("    else if constexpr (R == 4) cell_word4(a, h, sb, q, w, t);")
     print("    else if constexpr (R == 8) cell_word8(a, h, sb, q, w, t);")
     print("    else if constexpr (R == 16) cell_word16(a, h, sb, q, w, t);")
+    print('    else static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
+    for R in (1, 2, 4, 8, 16):
+        print(chunk_block(R, 8))
+    print("// cell_chunk{R}x8 by template argument: the 32 steps of a chunk")
+    print("template <int R>\n__device__ __forceinline__ void cell_chunk32(const int32_t *a, uint32_t *h,"
+          " const uint32_t *sb, uint32_t q, uint32_t &vt, uint32_t &qo)\n{")
+    print("    if constexpr (R == 1) cell_chunk1x8(a, h, sb, q, vt, qo);")
+    for R in (2, 4, 8, 16):
+        print(f"    else if constexpr (R == {R}) cell_chunk{R}x8(a, h, sb, q, vt, qo);")
     print('    else static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
     print("// cell_step{R}_byte{SEL} by template arguments")
     print("template <int R, int SEL>\n__device__ __forceinline__ void cell_step_byte(const int32_t *a, const uint32_t *h,"
