@@ -276,9 +276,11 @@ __device__ __forceinline__ void band_steps8(const int32_t (&a)[TM], uint32_t (&h
                                             uint32_t &v, uint32_t &Q, const uint32_t (&sym)[C / 4], int t0, int lane,
                                             int n)
 {
-// LCS_WORD_BLOCKS=0 runs one asm block per step instead of per symbol word
-// (the compiler then puts an s_nop 0 before every step: m = n = 2^20 took
-// 133.6 ms per comb against 132.1)
+// The compiler puts an s_nop 0 before every asm block that reads the strand,
+// so larger blocks pay it less often.  LCS_CHUNK_BLOCKS=0 falls back to one
+// block per symbol word and LCS_WORD_BLOCKS=0 then to one per step (m = n =
+// 2^20: 131.5 ms per comb with chunk blocks, 132.0 with word blocks on the
+// same box; 133.6 with step blocks against 132.1 for word blocks on another).
 #ifndef LCS_WORD_BLOCKS
 #define LCS_WORD_BLOCKS 1
 #endif
