@@ -165,6 +165,18 @@ template <int TM, int C, bool ROWMASK, bool COLMASK, bool PK = false>
 __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
                                            uint32_t &v, int32_t &bq, uint32_t &Q, int32_t &IB, int t0, int lane, int n)
 {
+#ifndef LCS_PACKED_CHUNK_BLOCKS
+#define LCS_PACKED_CHUNK_BLOCKS 1
+#endif
+    if constexpr (LCS_PACKED_CHUNK_BLOCKS && PK && !COLMASK && C == 32 && TM >= 4 && TM <= 16) {
+        // the packed chain's 32 steps, hand-offs included, in one asm block
+        // (cell_asm.h): no step waits out a hazard with an s_nop
+        uint32_t ua[TM];
+#pragma unroll
+        for (int q = 0; q < TM; ++q) ua[q] = (uint32_t)a[q];
+        cell16_chunk<TM>(ua, h, v, Q, IB, bq);
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < C; ++s) {
         const int32_t bin = IB;

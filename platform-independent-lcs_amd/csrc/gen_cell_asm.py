@@ -176,6 +176,63 @@ def chunk_block(R: int, W: int) -> str:
             f"    vt = {strand};\n    qo = {conv};\n}}\n")
 
 
+def packed_chunk(R: int, steps: int = 32) -> str:
+    """`steps` steps of the packed chain for R >= 3 rows as one block, the
+    hand-off included.  Per step: the symbol conveyor moves (IB one lane
+    down into a free register, lane 63 taking 0; bq one lane up in place of
+    the old IB, lane 0 keeping IB's lane 0), then the strand (v one lane up
+    into the register holding the conveyor's copy, lane 0 keeping the
+    conveyor's lane 0; the conveyor one lane down in place of v, lane 63
+    keeping v), then the rows as in packed_block.  The copy of the new
+    conveyor for the next step fills the last row's min -> bitop3 slot, and
+    the two symbol moves put two issues between the last bitop3 and the DPP
+    that reads its strand, so no step needs an s_nop.  Strand, conveyor and
+    copy rotate through three registers, IB, bq and the free one through
+    three others, both with period three."""
+    assert R >= 3
+    S, K, F = "va", "vb", "vc"        # strand, conveyor, copy of the conveyor
+    I, B, G = "ia", "ib", "ic"        # symbol conveyor IB, symbols bq, free
+    def X(k, sym): return f'"v_xor_b32_e32 %[r{k % 4}], %[a{k}], %[{sym}]\\n\\t"'
+    def Sb(k): return f'"v_pk_sub_u16 %[r{k % 4}], 1, %[r{k % 4}] op_sel_hi:[0,1] clamp\\n\\t"'
+    def M(k, hs): return f'"v_pk_mad_u16 %[r{k % 4}], %[r{k % 4}], -1, %[{hs}{k}] clamp\\n\\t"'
+    def N(k, hd, st): return f'"v_pk_min_u16 %[{hd}{k}], %[r{k % 4}], %[{st}]\\n\\t"'
+    def Bx(k, hs, hd, st): return f'"v_bitop3_b32 %[{st}], %[{hs}{k}], %[{st}], %[{hd}{k}] bitop3:0x96\\n\\t"'
+    ins = [f'"v_mov_b32_e32 %[{F}], %[{K}]\\n\\t"', '"s_nop 1\\n\\t"']
+    for st in range(steps):
+        hs, hd = ("h", "x") if st % 2 == 0 else ("x", "h")
+        ins += [f'"v_mov_b32_dpp %[{G}], %[{I}] wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\\n\\t"',
+                f'"v_mov_b32_dpp %[{I}], %[{B}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
+                f'"v_mov_b32_dpp %[{F}], %[{S}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
+                f'"v_mov_b32_dpp %[{S}], %[{K}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
+        # now: symbols bq in I, next IB in G, free B; strand in F, conveyor in S, free K
+        sym, strand, conv = I, F, S
+        ins += [X(0, sym), Sb(0), X(1, sym), M(0, hs), Sb(1), X(2, sym)]
+        for k in range(R):
+            ins.append(N(k, hd, strand))
+            if k + 1 < R:
+                ins.append(M(k + 1, hs))
+            else:
+                ins.append(f'"v_mov_b32_e32 %[{K}], %[{conv}]\\n\\t"')   # next step's copy of the conveyor
+            ins.append(Bx(k, hs, hd, strand))
+            if k + 3 < R:
+                ins.append(X(k + 3, sym))
+            if k + 2 < R:
+                ins.append(Sb(k + 2))
+        S, K, F = strand, conv, K
+        I, B, G = G, I, B
+    regs = {"va": "v", "vb": "q", "vc": "c", "ia": "ib", "ib": "bq", "ic": "g"}
+    outs = ['[va] "+v"(v)', '[vb] "+v"(q)', '[vc] "=&v"(c)', '[ia] "+v"(ib)', '[ib] "+v"(bq)', '[ic] "=&v"(g)'] + \
+           [f'[h{k}] "+v"(h[{k}])' for k in range(R)] + [f'[x{k}] "=&v"(x[{k}])' for k in range(R)] + \
+           [f'[r{q}] "=&v"(r{q})' for q in range(4)]
+    inp = [f'[a{k}] "v"(a[{k}])' for k in range(R)]
+    body = "\n        ".join(ins)
+    return (f"__device__ __forceinline__ void cell16_chunk{R}(const uint32_t *a, uint32_t *h, uint32_t &sv,"
+            f" uint32_t &sq, int32_t &sib, int32_t &sbq)\n{{\n"
+            f"    uint32_t v = sv, q = sq, c, ib = (uint32_t)sib, bq = (uint32_t)sbq, g, r0, r1, r2, r3, x[{R}];\n"
+            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n"
+            f"    sv = {regs[S]};\n    sq = {regs[K]};\n    sib = (int32_t){regs[I]};\n    sbq = (int32_t){regs[B]};\n}}\n")
+
+
 if __name__ == "__main__":
     print("// Generated by gen_cell_asm.py -- hand-scheduled cell blocks (see there).")
     print("#pragma once\n#include <stdint.h>\n")
@@ -201,6 +258,15 @@ if __name__ == "__main__":
         print(word_block(R))
     for R in (4, 8, 16):
         print(packed_block(R))
+    for R in (4, 8, 16):
+        print(packed_chunk(R))
+    print("// cell16_chunk{R} by template argument: the 32 steps of a packed chunk")
+    print("template <int R>\n__device__ __forceinline__ void cell16_chunk(const uint32_t *a, uint32_t *h, uint32_t &v,"
+          " uint32_t &q, int32_t &ib, int32_t &bq)\n{")
+    print("    if constexpr (R == 4) cell16_chunk4(a, h, v, q, ib, bq);")
+    print("    else if constexpr (R == 8) cell16_chunk8(a, h, v, q, ib, bq);")
+    print("    else if constexpr (R == 16) cell16_chunk16(a, h, v, q, ib, bq);")
+    print('    else static_assert(R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
     print("// cell16_block{R} by template argument")
     print("template <int R>\n__device__ __forceinline__ void cell16_block(const uint32_t *a, const uint32_t *h,"
           " uint32_t *hn, uint32_t sb, uint32_t &v)\n{")
