@@ -63,7 +63,7 @@ typedef struct lcs_schedule {
 /* The register chain reads b as bytes when every symbol of b lies in
  * [0, 255] (checked on the device per call): each lane then holds its own
  * column symbols and none move between lanes.  The bytes are staged (two
- * small launches) for pairs of 2^30 cells or more, and at any size with
+ * small launches) for pairs of 2^22 cells or more, and at any size with
  * LCS_VARIANT_CHAIN_BYTES; the results are the same either way. */
 #define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */
 /* Batches only: pairs of a batch with this variant are combed two per wave

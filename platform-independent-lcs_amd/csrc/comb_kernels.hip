@@ -369,6 +369,13 @@ __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int
             valid[k] = true;
             a[k] = (int32_t)sa;
             h[k] = hh;
+        } else if constexpr (S8) {
+            // rows past m hold a symbol no byte equals and strand 0: they
+            // never match and never swap (min(0, v) = 0), so every strand
+            // passes them unchanged and the band needs no row mask
+            valid[k] = r < m;
+            a[k] = valid[k] ? P.a[r] : -1;
+            h[k] = valid[k] ? (P.h_in ? (uint32_t)P.h_in[m - 1 - r] : (uint32_t)(m - 1 - r)) : 0u;
         } else {
             valid[k] = ROWMASK ? (r < m) : true;
             a[k] = valid[k] ? P.a[r] : 0;
@@ -575,9 +582,8 @@ __global__ __launch_bounds__(256) void comb_chain_kernel(ChainArgs A)
         // byte symbols when the pair has them and all of b fits in a byte
         const bool s8 = P.sym8 != nullptr && __builtin_amdgcn_readfirstlane(*P.sym8_bad) == 0;
         bool ok;
-        if (s8)
-            ok = rowmask ? band_pass<TM, C, true, false, PairDesc, true>(A, P, g, lane, (int)t)
-                         : band_pass<TM, C, false, false, PairDesc, true>(A, P, g, lane, (int)t);
+        if (s8)   // byte symbols pad a partial band's rows instead of masking them
+            ok = band_pass<TM, C, false, false, PairDesc, true>(A, P, g, lane, (int)t);
         else
             ok = rowmask ? band_pass<TM, C, true, false>(A, P, g, lane, (int)t)
                          : band_pass<TM, C, false, false>(A, P, g, lane, (int)t);

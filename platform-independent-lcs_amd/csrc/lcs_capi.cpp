@@ -348,21 +348,26 @@ int default_strips(const lcs_ctx *ctx, int32_t m, int32_t n, int tm)
 // kSym8MaxN, or a context whose budget cannot hold the 4(n + 256) bytes,
 // keeps the 32-bit symbols.  The staging is two small launches (about 8 us
 // of device time at n = 16384); a pair below kSym8MinCells cells, whose
-// comb that would not repay, skips it unless its schedule asks for
-// LCS_VARIANT_CHAIN_BYTES.
+// comb that would not repay it, skips it unless its schedule asks for
+// LCS_VARIANT_CHAIN_BYTES.  Besides the per-step saving, bytes let a
+// partial last band (m not a multiple of 64 * tile_m) pad its rows and run
+// the unmasked loop, where with 32-bit symbols it masks every cell and, as
+// the chain's last band, sets the pace of the whole launch (TM=1,
+// n = 103691: m = 1387 took 2.75 ms against 2.14 for the full-band
+// m = 1408; with bytes 1.84).
 #ifndef LCS_SYM8
 #define LCS_SYM8 1
 #endif
 constexpr int32_t kSym8MaxN = 1 << 30;
-constexpr double kSym8MinCells = 1073741824.0;   // 2^30
+constexpr double kSym8MinCells = 4194304.0;   // 2^22
 bool use_sym8(int variant, int32_t m, int32_t n)
 {
+    if (!LCS_SYM8 || n <= 0 || n > kSym8MaxN) return false;
     return variant == LCS_VARIANT_CHAIN_BYTES || (double)m * (double)n >= kSym8MinCells;
 }
 int stage_sym8(lcs_ctx *ctx, const int32_t *d_b, int32_t n, std::vector<PairDesc> &pairs,
                const std::vector<int32_t> &col0, hipStream_t stream)
 {
-    if (!LCS_SYM8 || n <= 0 || n > kSym8MaxN) return LCS_OK;
     const int32_t stride = lcsk_sym8_stride(n);
     const size_t bytes = (size_t)4 * 4 * (size_t)stride + 64;
     if (ctx->sym8.reserve(ctx, bytes) != LCS_OK) return LCS_OK;

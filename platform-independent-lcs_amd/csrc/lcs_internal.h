@@ -129,6 +129,9 @@ int launch_prepared(const PreparedLaunch &L, hipStream_t stream);
 int prepare_pairs_own_rings(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, DevBuf &descs,
                             unsigned *ctrl, PreparedLaunch &L, hipStream_t stream, int group_w = 1);
 int enqueue_pairs(lcs_ctx *ctx, std::vector<PairDesc> &pairs, int tm, int wps, hipStream_t stream, int group_w = 1);
+// whether a single pair's register chain stages its b as byte symbols
+// (lcsk_launch_sym8): pairs of 2^22 cells or more, or LCS_VARIANT_CHAIN_BYTES
+bool use_sym8(int variant, int32_t m, int32_t n);
 int default_batch_tile_m(int32_t m);
 bool valid_tile_m(int tm);
 int32_t bands_of(int32_t m, int32_t n, int tm);

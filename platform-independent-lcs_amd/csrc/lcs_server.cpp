@@ -108,9 +108,14 @@ struct Request {
     Clock::time_point t_submit, t_start, t_done;
 };
 
-size_t footprint(int32_t m, int32_t n)
+// Bytes of a request's arena block: a, b, h, v, the ring and, for a pair
+// whose register chain takes byte symbols (use_sym8), the byte copies of b
+// and their range flag, each part kAlign-aligned.
+size_t sym8_bytes(int32_t n) { return (size_t)16 * (size_t)lcsk_sym8_stride(n) + kAlign; }
+size_t footprint(int32_t m, int32_t n, int variant)
 {
-    return (size_t)4 * (2 * (size_t)m + 2 * (size_t)n) + 8 * lcsk_ring_words(n) + 4 * kAlign;
+    return (size_t)4 * (2 * (size_t)m + 2 * (size_t)n) + 8 * lcsk_ring_words(n) + 4 * kAlign +
+           (use_sym8(variant, m, n) ? sym8_bytes(n) + kAlign : 0);
 }
 
 struct Slot {
@@ -206,6 +211,17 @@ int launch_slot(lcs_server *S, Slot &slot)
         P.m = m;
         P.n = n;
         P.E = E;  // prepare_pairs assigns rings from `ring`; each pair's own lies right here
+        if (c < kNumTiles && use_sym8(R->sched.variant, m, n) && R->bytes >= footprint(m, n, R->sched.variant)) {
+            // byte symbols (register chain): the copies follow the ring in the block
+            const int32_t stride = lcsk_sym8_stride(n);
+            uint32_t *copies = (uint32_t *)(((uintptr_t)(E + lcsk_ring_words(n)) + kAlign - 1) & ~(uintptr_t)(kAlign - 1));
+            int32_t *bad = (int32_t *)(copies + (size_t)4 * stride);
+            if (lcsk_launch_sym8(db, n, copies, stride, bad, slot.stream)) return LCS_EIO;
+            P.sym8 = copies;
+            P.sym8_bad = bad;
+            P.sym8_col0 = 0;
+            P.sym8_stride = stride;
+        }
         groups[(size_t)c].push_back(P);
     }
     // every class's descriptors go up in one copy before any kernel is
@@ -476,7 +492,7 @@ int lcs_server_submit(lcs_server *S, const int32_t *a, int32_t m, const int32_t 
         std::memcpy(R->feat, features, sizeof(R->feat));
         R->has_feat = true;
     }
-    R->bytes = footprint(m, n);
+    R->bytes = footprint(m, n, R->sched.variant);
     if (R->bytes > S->budget) return LCS_ENOMEM;  // could never be admitted
     R->t_submit = Clock::now();
     {
