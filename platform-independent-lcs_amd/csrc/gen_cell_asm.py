@@ -145,18 +145,29 @@ def chunk_block(R: int, W: int) -> str:
     one of t, u, w that holds neither the strand nor the conveyor (the
     conveyor's copy goes there first), so the roles rotate with period three;
     the rows' h strands alternate between h and a scratch set (4W is even,
-    so they end in h).  Returns the strand in `vt` and the conveyor in `qo`."""
+    so they end in h).  Returns the strand in `vt` and the conveyor in `qo`.
+
+    With two rows or more the hand-off is two instructions: vcc holds lane 0
+    alone, and one `v_cndmask_b32_dpp` (wave_shr:1 with bound_ctrl, so lane
+    0 reads 0 and the mask picks the conveyor's lane 0 instead) builds the
+    shifted strand; the two compares before it cover the DPP read's two wait
+    states.  One row keeps the copy and two plain DPP moves, the copy filling
+    the wait state that the single compare leaves open."""
     def cmp(sel, word, s, a):
         return f'"v_cmp_eq_u32_sdwa %[{s}], %[{a}], %[b{word}] src0_sel:DWORD src1_sel:BYTE_{sel}\\n\\t"'
-    ins = []
+    ins = ['"s_mov_b64 vcc, 1\\n\\t"'] if R > 1 else []
     strand, conv = "w", "q"           # before step 0: strand in w (the input v), conveyor in q
     for k in range(4 * W):
         word, sel = divmod(k, 4)
         dest = "t" if k == 0 else ({"t", "u", "w"} - {strand, conv}).pop()
         src_h, dst_h = ("h", "x") if k % 2 == 0 else ("x", "h")
-        ins += [cmp(sel, word, "s0", "a0")] + ([cmp(sel, word, "s1", "a1")] if R > 1 else []) + [
-            f'"v_mov_b32_e32 %[{dest}], %[{conv}]\\n\\t"',
-            f'"v_mov_b32_dpp %[{dest}], %[{strand}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
+        if R > 1:
+            hand = [f'"v_cndmask_b32_dpp %[{dest}], %[{strand}], %[{conv}], vcc wave_shr:1 row_mask:0xf'
+                    f' bank_mask:0xf bound_ctrl:1\\n\\t"']
+        else:
+            hand = [f'"v_mov_b32_e32 %[{dest}], %[{conv}]\\n\\t"',
+                    f'"v_mov_b32_dpp %[{dest}], %[{strand}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"']
+        ins += [cmp(sel, word, "s0", "a0")] + ([cmp(sel, word, "s1", "a1")] if R > 1 else []) + hand + [
             f'"v_mov_b32_dpp %[{strand}], %[{conv}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
         strand, conv = dest, strand
         for r in range(R):
@@ -170,9 +181,11 @@ def chunk_block(R: int, W: int) -> str:
            [f'[x{r}] "=&v"(x[{r}])' for r in range(R)] + ['[s0] "=&s"(s0)', '[s1] "=&s"(s1)']
     inp = [f'[b{i}] "v"(sb[{i}])' for i in range(W)] + ['[q] "v"(q)'] + [f'[a{r}] "v"(a[{r}])' for r in range(R)]
     body = "\n        ".join(ins)
+    clob = ' : "vcc"' if R > 1 else ''
     return (f"__device__ __forceinline__ void cell_chunk{R}x{W}(const int32_t *a, uint32_t *h, const uint32_t *sb,"
             f" uint32_t q, uint32_t &vt, uint32_t &qo)\n{{\n    uint64_t s0, s1;\n    uint32_t w = vt, t, u, x[{R}];\n"
-            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n"
+            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)}"
+            f"{clob});\n"
             f"    vt = {strand};\n    qo = {conv};\n}}\n")
 
 
