@@ -191,41 +191,47 @@ def chunk_block(R: int, W: int) -> str:
 
 def packed_chunk(R: int, steps: int = 32) -> str:
     """`steps` steps of the packed chain for R >= 3 rows as one block, the
-    hand-off included.  Per step: the symbol conveyor moves (IB one lane
-    down into a free register, lane 63 taking 0; bq one lane up in place of
-    the old IB, lane 0 keeping IB's lane 0), then the strand (v one lane up
-    into the register holding the conveyor's copy, lane 0 keeping the
-    conveyor's lane 0; the conveyor one lane down in place of v, lane 63
-    keeping v), then the rows as in packed_block.  The copy of the new
-    conveyor for the next step fills the last row's min -> bitop3 slot, and
-    the two symbol moves put two issues between the last bitop3 and the DPP
-    that reads its strand, so no step needs an s_nop.  Strand, conveyor and
-    copy rotate through three registers, IB, bq and the free one through
-    three others, both with period three."""
+    hand-off included.  Per step: the symbols (bq one lane up into IB's
+    register, lane 0 keeping IB's lane 0; the next step's IB, IB one lane
+    down with lane 63 taking 0, was made during the previous step), then the
+    strand (one `v_cndmask_b32_dpp` with wave_shr:1 and bound_ctrl moves v
+    one lane up into the free register, vcc holding lane 0 alone so that
+    lane 0 takes the conveyor's lane 0; the conveyor one lane down in place
+    of v, lane 63 keeping v), then the rows as in packed_block.  The next
+    step's IB move fills the last row's min -> bitop3 slot (an s_nop in the
+    chunk's last step, whose IB the caller may reload), and the symbol move
+    and the first xor put two issues between the last bitop3 and the DPP
+    that reads its strand, so no other step needs an s_nop.  Strand,
+    conveyor and the free register rotate through three registers, IB, bq
+    and the free one through three others, both with period three."""
     assert R >= 3
-    S, K, F = "va", "vb", "vc"        # strand, conveyor, copy of the conveyor
+    S, K, F = "va", "vb", "vc"        # strand, conveyor, free
     I, B, G = "ia", "ib", "ic"        # symbol conveyor IB, symbols bq, free
     def X(k, sym): return f'"v_xor_b32_e32 %[r{k % 4}], %[a{k}], %[{sym}]\\n\\t"'
     def Sb(k): return f'"v_pk_sub_u16 %[r{k % 4}], 1, %[r{k % 4}] op_sel_hi:[0,1] clamp\\n\\t"'
     def M(k, hs): return f'"v_pk_mad_u16 %[r{k % 4}], %[r{k % 4}], -1, %[{hs}{k}] clamp\\n\\t"'
     def N(k, hd, st): return f'"v_pk_min_u16 %[{hd}{k}], %[r{k % 4}], %[{st}]\\n\\t"'
     def Bx(k, hs, hd, st): return f'"v_bitop3_b32 %[{st}], %[{hs}{k}], %[{st}], %[{hd}{k}] bitop3:0x96\\n\\t"'
-    ins = [f'"v_mov_b32_e32 %[{F}], %[{K}]\\n\\t"', '"s_nop 1\\n\\t"']
+    def ib_next(dst, src): return f'"v_mov_b32_dpp %[{dst}], %[{src}] wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\\n\\t"'
+    ins = ['"s_mov_b64 vcc, 1\\n\\t"', '"s_nop 1\\n\\t"', ib_next(G, I)]
     for st in range(steps):
         hs, hd = ("h", "x") if st % 2 == 0 else ("x", "h")
-        ins += [f'"v_mov_b32_dpp %[{G}], %[{I}] wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\\n\\t"',
-                f'"v_mov_b32_dpp %[{I}], %[{B}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
-                f'"v_mov_b32_dpp %[{F}], %[{S}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"',
+        # symbols bq move into I (lane 0 keeps IB's lane 0); the next IB is already in G
+        ins += [f'"v_mov_b32_dpp %[{I}], %[{B}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"']
+        sym = I
+        ins += [X(0, sym),
+                f'"v_cndmask_b32_dpp %[{F}], %[{S}], %[{K}], vcc wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\\n\\t"',
                 f'"v_mov_b32_dpp %[{S}], %[{K}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
-        # now: symbols bq in I, next IB in G, free B; strand in F, conveyor in S, free K
-        sym, strand, conv = I, F, S
-        ins += [X(0, sym), Sb(0), X(1, sym), M(0, hs), Sb(1), X(2, sym)]
+        strand, conv = F, S
+        ins += [Sb(0), X(1, sym), M(0, hs), Sb(1), X(2, sym)]
         for k in range(R):
             ins.append(N(k, hd, strand))
             if k + 1 < R:
                 ins.append(M(k + 1, hs))
+            elif st + 1 < steps:
+                ins.append(ib_next(B, G))   # the next step's IB (G one lane down) into the free register
             else:
-                ins.append(f'"v_mov_b32_e32 %[{K}], %[{conv}]\\n\\t"')   # next step's copy of the conveyor
+                ins.append('"s_nop 0\\n\\t"')
             ins.append(Bx(k, hs, hd, strand))
             if k + 3 < R:
                 ins.append(X(k + 3, sym))
@@ -242,7 +248,7 @@ def packed_chunk(R: int, steps: int = 32) -> str:
     return (f"__device__ __forceinline__ void cell16_chunk{R}(const uint32_t *a, uint32_t *h, uint32_t &sv,"
             f" uint32_t &sq, int32_t &sib, int32_t &sbq)\n{{\n"
             f"    uint32_t v = sv, q = sq, c, ib = (uint32_t)sib, bq = (uint32_t)sbq, g, r0, r1, r2, r3, x[{R}];\n"
-            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)});\n"
+            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)} : \"vcc\");\n"
             f"    sv = {regs[S]};\n    sq = {regs[K]};\n    sib = (int32_t){regs[I]};\n    sbq = (int32_t){regs[B]};\n}}\n")
 
 
