@@ -151,8 +151,9 @@ def chunk_block(R: int, W: int) -> str:
     alone, and one `v_cndmask_b32_dpp` (wave_shr:1 with bound_ctrl, so lane
     0 reads 0 and the mask picks the conveyor's lane 0 instead) builds the
     shifted strand; the two compares before it cover the DPP read's two wait
-    states.  One row keeps the copy and two plain DPP moves, the copy filling
-    the wait state that the single compare leaves open."""
+    states.  One row turns the pair round (chunk_block1)."""
+    if R == 1:
+        return chunk_block1(W)
     def cmp(sel, word, s, a):
         return f'"v_cmp_eq_u32_sdwa %[{s}], %[{a}], %[b{word}] src0_sel:DWORD src1_sel:BYTE_{sel}\\n\\t"'
     ins = ['"s_mov_b64 vcc, 1\\n\\t"'] if R > 1 else []
@@ -186,6 +187,40 @@ def chunk_block(R: int, W: int) -> str:
             f" uint32_t q, uint32_t &vt, uint32_t &qo)\n{{\n    uint64_t s0, s1;\n    uint32_t w = vt, t, u, x[{R}];\n"
             f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)}"
             f"{clob});\n"
+            f"    vt = {strand};\n    qo = {conv};\n}}\n")
+
+
+def chunk_block1(W: int) -> str:
+    """chunk_block for one row.  The single compare cannot cover the two
+    wait states between the row's bitop3 and a DPP read of its strand, so
+    the conveyor moves first: one `v_cndmask_b32_dpp` shifts it one lane down
+    into the free register, vcc holding lane 63 alone so that lane 63 takes
+    the strand (a plain operand, which has no wait state); then the compare;
+    then the strand moves one lane up in place of the old conveyor, whose
+    lane 0 (the band input) the DPP keeps.  Three VALU per step besides the
+    cell's three; strand, conveyor and free register rotate through w, q and
+    t with period three."""
+    def cmp(sel, word):
+        return f'"v_cmp_eq_u32_sdwa %[s0], %[a0], %[b{word}] src0_sel:DWORD src1_sel:BYTE_{sel}\\n\\t"'
+    ins = ['"s_mov_b32 vcc_lo, 0\\n\\t"', '"s_mov_b32 vcc_hi, 0x80000000\\n\\t"']
+    strand, conv, free = "w", "q", "t"
+    for k in range(4 * W):
+        word, sel = divmod(k, 4)
+        src_h, dst_h = ("h", "x") if k % 2 == 0 else ("x", "h")
+        move_conv = (f'"v_cndmask_b32_dpp %[{free}], %[{conv}], %[{strand}], vcc wave_shl:1 row_mask:0xf'
+                     f' bank_mask:0xf bound_ctrl:1\\n\\t"')
+        move_strand = f'"v_mov_b32_dpp %[{conv}], %[{strand}] wave_shr:1 row_mask:0xf bank_mask:0xf\\n\\t"'
+        ins += ([cmp(sel, word), move_conv] if k == 0 else [move_conv, cmp(sel, word)]) + [move_strand]
+        strand, conv, free = conv, free, strand
+        ins.append(f'"v_min_u32_e32 %[{dst_h}0], %[{src_h}0], %[{strand}]\\n\\t"')
+        ins.append(f'"v_cndmask_b32_e64 %[{dst_h}0], %[{dst_h}0], %[{strand}], %[s0]\\n\\t"')
+        ins.append(f'"v_bitop3_b32 %[{strand}], %[{src_h}0], %[{strand}], %[{dst_h}0] bitop3:0x96\\n\\t"')
+    outs = ['[w] "+v"(w)', '[q] "+v"(q)', '[t] "=&v"(t)', '[h0] "+v"(h[0])', '[x0] "=&v"(x0)', '[s0] "=&s"(s0)']
+    inp = [f'[b{i}] "v"(sb[{i}])' for i in range(W)] + ['[a0] "v"(a[0])']
+    body = "\n        ".join(ins)
+    return (f"__device__ __forceinline__ void cell_chunk1x{W}(const int32_t *a, uint32_t *h, const uint32_t *sb,"
+            f" uint32_t q, uint32_t &vt, uint32_t &qo)\n{{\n    uint64_t s0;\n    uint32_t w = vt, t, x0;\n"
+            f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)} : \"vcc\");\n"
             f"    vt = {strand};\n    qo = {conv};\n}}\n")
 
 
