@@ -48,6 +48,7 @@ struct CoupleDesc {
     int32_t m_[2], n_[2];
     unsigned long long *E;       // 2*n ring words (tag << 32 | Y strand << 16 | X strand) + 64 trash words
     int32_t m, n, nbands, tile_m;
+    int32_t onehot;              // both pairs' symbols in [0, 15]: one-hot cells (1 << symbol per half)
 };
 
 // Words of ring a pair of n columns needs.
@@ -78,8 +79,9 @@ struct ChainArgs {
 int lcsk_launch_chain(const ChainArgs &args, const PairDesc *d_pairs, int npairs, int max_n, int tile_m,
                       int grid_blocks, int block_threads, hipStream_t stream);
 int lcsk_launch_identity(int32_t *h_out, int32_t m, int32_t *v_out, int32_t n, hipStream_t stream);
-// A string whose symbols the packed chain needs in [0, 0xfffd]: bad[item]
-// becomes nonzero when one is outside.
+// A string whose symbols the packed chain needs in [0, 0xfffd]: bit 0 of
+// bad[item] is set when one is outside, bit 1 when one is above 15 (the
+// one-hot cells then cannot take the pair).
 struct SymbolRange {
     const int32_t *x;
     int32_t len, item;

@@ -154,6 +154,20 @@ __device__ __forceinline__ void cell16(uint32_t sa, uint32_t sb, uint32_t &h, ui
     h = hn;
 }
 
+// The packed cell with one-hot symbols (each half holds 1 << symbol, 0 for
+// padding; alphabets of at most 16): sa & sb is nonzero exactly on a match,
+// and sat(mk * 0xffff + h) is 0xffff there, so the match mask costs one
+// v_and instead of v_xor + v_pk_sub_u16: 4 VALU for 2 cells.
+__device__ __forceinline__ void cell16_oh(uint32_t sa, uint32_t sb, uint32_t &h, uint32_t &v)
+{
+    uint32_t hx;
+    asm("v_pk_mad_u16 %0, %1, -1, %2 clamp" : "=v"(hx) : "v"(sa & sb), "v"(h));
+    const uint32_t hn = __builtin_bit_cast(
+        uint32_t, __builtin_elementwise_min(__builtin_bit_cast(lcsk_u16x2, hx), __builtin_bit_cast(lcsk_u16x2, v)));
+    v = __builtin_amdgcn_bitop3_b32(h, v, hn, 0x96);
+    h = hn;
+}
+
 // Steps t0 .. t0+C-1 of a band.  COLMASK: some lane's column leaves [0, n).
 // Q (strands) and IB (symbols) are the conveyors.  On entry Q's lanes
 // 0 .. C-1 hold the chunk's inputs (the input of step t0+s in lane s); each
@@ -161,7 +175,7 @@ __device__ __forceinline__ void cell16(uint32_t sa, uint32_t sb, uint32_t &h, ui
 // 63's strand enters at the top, so on return Q's lanes 64-C .. 63 hold the
 // chunk's outputs (columns t0-64 .. t0-64+C-1).  One shift serves both
 // directions: C <= 64, so the inputs and outputs never meet.
-template <int TM, int C, bool ROWMASK, bool COLMASK, bool PK = false>
+template <int TM, int C, bool ROWMASK, bool COLMASK, bool PK = false, bool OH = false>
 __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)[TM], const bool (&valid)[TM],
                                            uint32_t &v, int32_t &bq, uint32_t &Q, int32_t &IB, int t0, int lane, int n)
 {
@@ -174,7 +188,8 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
         uint32_t ua[TM];
 #pragma unroll
         for (int q = 0; q < TM; ++q) ua[q] = (uint32_t)a[q];
-        cell16_chunk<TM>(ua, h, v, Q, IB, bq);
+        if constexpr (OH) cell16oh_chunk<TM>(ua, h, v, Q, IB, bq);
+        else cell16_chunk<TM>(ua, h, v, Q, IB, bq);
         return;
     }
 #pragma unroll
@@ -191,7 +206,10 @@ __device__ __forceinline__ void band_steps(const int32_t (&a)[TM], uint32_t (&h)
             active = (j >= 0) && (j < n);
         }
         if (active) {
-            if constexpr (PK && !COLMASK && TM >= 4) {
+            if constexpr (PK && OH) {
+#pragma unroll
+                for (int k = 0; k < TM; ++k) cell16_oh((uint32_t)a[k], (uint32_t)bq, h[k], v);
+            } else if constexpr (PK && !COLMASK && TM >= 4) {
                 // software-pipelined rows (cell_asm.h): no packed result is
                 // read by the next issue, so no s_nop between the rows
                 constexpr int R = TM >= 16 ? 16 : TM;
@@ -344,7 +362,7 @@ constexpr uint32_t kPadRow16 = 0xffffu, kPadCol16 = 0xfffeu;
 // b; every lane loads its own C symbols of the next chunk (C/4 words from
 // the copy that starts its window on a word boundary) and no symbol
 // conveyor runs.
-template <int TM, int C, bool ROWMASK, bool PK, typename Desc, bool S8 = false>
+template <int TM, int C, bool ROWMASK, bool PK, typename Desc, bool S8 = false, bool OH = false>
 __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int g, int lane, int t_id)
 {
     const int m = P.m, n = P.n;
@@ -361,7 +379,9 @@ __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 const bool in = r < P.m_[q];
-                const uint32_t sym = in ? ((uint32_t)P.a[q][r] & 0xffffu) : kPadRow16;
+                uint32_t sym;
+                if constexpr (OH) sym = in ? 1u << ((uint32_t)P.a[q][r] & 15u) : 0u;
+                else sym = in ? ((uint32_t)P.a[q][r] & 0xffffu) : kPadRow16;
                 const uint32_t id = in ? (uint32_t)(P.m_[q] - 1 - r) : 0u;
                 sa |= sym << (16 * q);
                 hh |= id << (16 * q);
@@ -394,8 +414,9 @@ __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int
             uint32_t x = 0;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                const uint32_t sym = (uint32_t)((gsym *)P.b[q])[min((int)c, P.n_[q] - 1)] & 0xffffu;
-                x |= ((int)c < P.n_[q] ? sym : kPadCol16) << (16 * q);
+                const uint32_t raw = (uint32_t)((gsym *)P.b[q])[min((int)c, P.n_[q] - 1)];
+                if constexpr (OH) x |= ((int)c < P.n_[q] ? 1u << (raw & 15u) : 0u) << (16 * q);
+                else x |= ((int)c < P.n_[q] ? raw & 0xffffu : kPadCol16) << (16 * q);
             }
             return (int32_t)x;
         } else {
@@ -522,7 +543,7 @@ __device__ __forceinline__ bool band_pass(const ChainArgs &A, const Desc &P, int
         if constexpr (S8)
             band_steps8<TM, C, ROWMASK, decltype(colmask)::value>(a, h, valid, v, Q, sym, t0, lane, n);
         else
-            band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK>(a, h, valid, v, bq, Q, IB, t0, lane, n);
+            band_steps<TM, C, ROWMASK, decltype(colmask)::value, PK, OH>(a, h, valid, v, bq, Q, IB, t0, lane, n);
         Osave = Q;
         if (!decltype(colmask)::value || k + 1 < nchunks) {   // an interior chunk is never the last
             if constexpr (S8) {
@@ -604,7 +625,10 @@ __global__ __launch_bounds__(256) void comb_chain16_kernel(ChainArgs A)
         if (__builtin_amdgcn_readfirstlane(ld_flag(A.abort_flag))) break;
         const int2 task = A.tasks[t];
         const CoupleDesc D = A.couples[task.x];
-        if (!band_pass<TM, C, false, true>(A, D, task.y, lane, (int)t)) break;
+        const bool ok = __builtin_amdgcn_readfirstlane(D.onehot)
+                            ? band_pass<TM, C, false, true, CoupleDesc, false, true>(A, D, task.y, lane, (int)t)
+                            : band_pass<TM, C, false, true>(A, D, task.y, lane, (int)t);
+        if (!ok) break;
     }
 }
 
@@ -975,15 +999,21 @@ __global__ void comb_sym8_kernel(const int32_t *b, int32_t n, uint32_t *out, int
     }
 }
 
-// Range check of the packed chain's symbols (blockIdx.y = string).
+// Range check of the packed chain's symbols (blockIdx.y = string): bit 0 of
+// bad[item] for a symbol outside [0, 0xfffd], bit 1 for one above 15 (the
+// one-hot cells need every symbol in [0, 15]).
 __global__ void check16_kernel(const SymbolRange *ranges, int32_t *bad)
 {
     const SymbolRange R = ranges[blockIdx.y];
-    bool out = false;
+    bool out = false, wide = false;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < R.len;
-         i += (long long)gridDim.x * blockDim.x)
-        out = out || (uint32_t)R.x[i] > 0xfffdu;
-    if (__builtin_amdgcn_ballot_w64(out) != 0 && (threadIdx.x & 63) == 0) atomicOr(&bad[R.item], 1);
+         i += (long long)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)R.x[i];
+        out = out || x > 0xfffdu;
+        wide = wide || x > 15u;
+    }
+    const int flags = (__builtin_amdgcn_ballot_w64(out) != 0 ? 1 : 0) | (__builtin_amdgcn_ballot_w64(wide) != 0 ? 2 : 0);
+    if (flags && (threadIdx.x & 63) == 0) atomicOr(&bad[R.item], flags);
 }
 
 // v_out = the strands the last band left in its slot (blockIdx.y = pair).

@@ -224,7 +224,7 @@ def chunk_block1(W: int) -> str:
             f"    vt = {strand};\n    qo = {conv};\n}}\n")
 
 
-def packed_chunk(R: int, steps: int = 32) -> str:
+def packed_chunk(R: int, steps: int = 32, oh: bool = False) -> str:
     """`steps` steps of the packed chain for R >= 3 rows as one block, the
     hand-off included.  Per step: the symbols (bq one lane up into IB's
     register, lane 0 keeping IB's lane 0; the next step's IB, IB one lane
@@ -238,11 +238,19 @@ def packed_chunk(R: int, steps: int = 32) -> str:
     and the first xor put two issues between the last bitop3 and the DPP
     that reads its strand, so no other step needs an s_nop.  Strand,
     conveyor and the free register rotate through three registers, IB, bq
-    and the free one through three others, both with period three."""
+    and the free one through three others, both with period three.
+
+    oh: one-hot symbols (each 16-bit half holds 1 << symbol, padding 0, for
+    alphabets of at most 16).  A row's match mask is then a & b, nonzero on
+    a match, and sat(mk * 0xffff + h) is 0xffff there: four VALU per row
+    (v_and, v_pk_mad_u16 clamp, v_pk_min_u16, v_bitop3) in the order
+    N(k) M(k+1) B(k) A(k+2), every packed result again read two issues
+    later."""
     assert R >= 3
     S, K, F = "va", "vb", "vc"        # strand, conveyor, free
     I, B, G = "ia", "ib", "ic"        # symbol conveyor IB, symbols bq, free
-    def X(k, sym): return f'"v_xor_b32_e32 %[r{k % 4}], %[a{k}], %[{sym}]\\n\\t"'
+    op = "v_and_b32_e32" if oh else "v_xor_b32_e32"
+    def X(k, sym): return f'"{op} %[r{k % 4}], %[a{k}], %[{sym}]\\n\\t"'
     def Sb(k): return f'"v_pk_sub_u16 %[r{k % 4}], 1, %[r{k % 4}] op_sel_hi:[0,1] clamp\\n\\t"'
     def M(k, hs): return f'"v_pk_mad_u16 %[r{k % 4}], %[r{k % 4}], -1, %[{hs}{k}] clamp\\n\\t"'
     def N(k, hd, st): return f'"v_pk_min_u16 %[{hd}{k}], %[r{k % 4}], %[{st}]\\n\\t"'
@@ -258,7 +266,10 @@ def packed_chunk(R: int, steps: int = 32) -> str:
                 f'"v_cndmask_b32_dpp %[{F}], %[{S}], %[{K}], vcc wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\\n\\t"',
                 f'"v_mov_b32_dpp %[{S}], %[{K}] wave_shl:1 row_mask:0xf bank_mask:0xf\\n\\t"']
         strand, conv = F, S
-        ins += [Sb(0), X(1, sym), M(0, hs), Sb(1), X(2, sym)]
+        if oh:
+            ins += [M(0, hs), X(1, sym)]
+        else:
+            ins += [Sb(0), X(1, sym), M(0, hs), Sb(1), X(2, sym)]
         for k in range(R):
             ins.append(N(k, hd, strand))
             if k + 1 < R:
@@ -268,6 +279,10 @@ def packed_chunk(R: int, steps: int = 32) -> str:
             else:
                 ins.append('"s_nop 0\\n\\t"')
             ins.append(Bx(k, hs, hd, strand))
+            if oh:
+                if k + 2 < R:
+                    ins.append(X(k + 2, sym))
+                continue
             if k + 3 < R:
                 ins.append(X(k + 3, sym))
             if k + 2 < R:
@@ -280,7 +295,8 @@ def packed_chunk(R: int, steps: int = 32) -> str:
            [f'[r{q}] "=&v"(r{q})' for q in range(4)]
     inp = [f'[a{k}] "v"(a[{k}])' for k in range(R)]
     body = "\n        ".join(ins)
-    return (f"__device__ __forceinline__ void cell16_chunk{R}(const uint32_t *a, uint32_t *h, uint32_t &sv,"
+    name = f"cell16oh_chunk{R}" if oh else f"cell16_chunk{R}"
+    return (f"__device__ __forceinline__ void {name}(const uint32_t *a, uint32_t *h, uint32_t &sv,"
             f" uint32_t &sq, int32_t &sib, int32_t &sbq)\n{{\n"
             f"    uint32_t v = sv, q = sq, c, ib = (uint32_t)sib, bq = (uint32_t)sbq, g, r0, r1, r2, r3, x[{R}];\n"
             f"    asm volatile(\n        {body}\n        : {', '.join(outs)}\n        : {', '.join(inp)} : \"vcc\");\n"
@@ -320,6 +336,15 @@ if __name__ == "__main__":
     print("    if constexpr (R == 4) cell16_chunk4(a, h, v, q, ib, bq);")
     print("    else if constexpr (R == 8) cell16_chunk8(a, h, v, q, ib, bq);")
     print("    else if constexpr (R == 16) cell16_chunk16(a, h, v, q, ib, bq);")
+    print('    else static_assert(R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
+    for R in (4, 8, 16):
+        print(packed_chunk(R, oh=True))
+    print("// cell16oh_chunk{R} by template argument: the 32 steps of a packed chunk, one-hot symbols")
+    print("template <int R>\n__device__ __forceinline__ void cell16oh_chunk(const uint32_t *a, uint32_t *h, uint32_t &v,"
+          " uint32_t &q, int32_t &ib, int32_t &bq)\n{")
+    print("    if constexpr (R == 4) cell16oh_chunk4(a, h, v, q, ib, bq);")
+    print("    else if constexpr (R == 8) cell16oh_chunk8(a, h, v, q, ib, bq);")
+    print("    else if constexpr (R == 16) cell16oh_chunk16(a, h, v, q, ib, bq);")
     print('    else static_assert(R == 4 || R == 8 || R == 16, "rows per block");\n}\n')
     print("// cell16_block{R} by template argument")
     print("template <int R>\n__device__ __forceinline__ void cell16_block(const uint32_t *a, const uint32_t *h,"

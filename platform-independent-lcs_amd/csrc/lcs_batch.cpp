@@ -14,6 +14,11 @@
 #include "lcs_hip.h"
 #include "lcs_internal.h"
 
+// LCS_ONEHOT=0 keeps every couple on the xor cells (A/B timing of the one-hot cells).
+#ifndef LCS_ONEHOT
+#define LCS_ONEHOT 1
+#endif
+
 namespace lcs {
 
 // A batch is grouped by tile height and variant (one persistent launch per
@@ -297,6 +302,7 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
         if (hipMemcpyAsync(bad.data(), d_bad, bad.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return LCS_EIO;
+        std::vector<char> wide_of((size_t)npairs, 0);
         for (int c = 0; c < kNumTiles; ++c) {
             // candidates of this tile height ordered by (bands, columns, pair):
             // one 64-bit key each, so the sort compares integers
@@ -305,13 +311,17 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
             for (size_t i = 0; i < cand.size(); ++i) {
                 if (cand_tm[i] != c) continue;
                 const int32_t p = cand[i];
-                if (bad[i]) {
+                if (bad[i] & 1) {
                     groups[(size_t)c].push_back(desc(p));
                     continue;
                 }
-                // bands < 2^10 and n <= 65535 (packable), p < 2^31
+                // bands < 2^10 and n <= 65535 (packable), p < 2^31; pairs with
+                // a symbol above 15 sort after the rest, so one-hot pairs couple
+                // with each other
                 const unsigned long long bands = (unsigned long long)bands_of(m[p], n[p], tm);
-                keyed.push_back(bands << 48 | (unsigned long long)n[p] << 32 | (uint32_t)p);
+                const unsigned long long wide = (bad[i] & 2) ? 1ull : 0ull;
+                keyed.push_back(wide << 63 | bands << 48 | (unsigned long long)n[p] << 32 | (uint32_t)p);
+                wide_of[(size_t)p] = (char)wide;
             }
             std::sort(keyed.begin(), keyed.end());
             std::vector<int32_t> mine(keyed.size());
@@ -330,6 +340,7 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
                 }
                 D.m = std::max(D.m_[0], D.m_[1]);
                 D.n = std::max(D.n_[0], D.n_[1]);
+                D.onehot = (LCS_ONEHOT && !wide_of[(size_t)mine[i]] && !wide_of[(size_t)mine[i + 1]]) ? 1 : 0;
                 couples[(size_t)(2 * kNumTiles + c)].push_back(D);
             }
             if (i < mine.size()) groups[(size_t)c].push_back(desc(mine[i]));

@@ -104,6 +104,33 @@ def test_packed16_batch_matches_oracle(ctx):
         assert np.array_equal(h, ho) and np.array_equal(v, vo), (i, len(a), len(b))
 
 
+def test_packed16_onehot_and_xor_cells_match_oracle(ctx):
+    """Couples whose symbols all lie in [0, 15] run the one-hot packed cells
+    (1 << symbol per half), the others the xor cells: alphabets of exactly 16
+    (symbol 15 present) and 17 (symbol 16 present, so the pair is refused by
+    the one-hot cells), a batch mixing both at every tile height so that one
+    couple joins a one-hot pair with a wide one, uneven shapes (padding rows
+    and columns carry 0 in the one-hot encoding), and long runs of a single
+    symbol; every pair equals the oracle."""
+    rng = np.random.default_rng(1616)
+    a_list, b_list, tms = [], [], []
+    for tm in (1, 2, 4, 8, 16):
+        for k, sigma in enumerate((16, 17, 4, 16, 17, 2, 16)):
+            m = int(rng.integers(40, 2500)); n = int(rng.integers(40, 2500))
+            a = rng.integers(0, sigma, m).astype(np.int32)
+            b = rng.integers(0, sigma, n).astype(np.int32)
+            a[0] = sigma - 1                     # the top symbol is present
+            if k == 5:                           # long single-symbol runs
+                a[: m // 2] = 1
+                b[n // 3:] = 1
+            a_list.append(a); b_list.append(b); tms.append(tm)
+    scheds = [lcs_amd.schedule(tile_m=t, variant=PACKED16) for t in tms]
+    hs, vs = ctx.batch_comb(a_list, b_list, scheds=scheds)
+    for i, (a, b, h, v) in enumerate(zip(a_list, b_list, hs, vs)):
+        ho, vo = oracle.comb(a, b, order="simd")
+        assert np.array_equal(h, ho) and np.array_equal(v, vo), (i, tms[i], len(a), len(b), int(a.max()))
+
+
 def test_packed16_large_batch_equals_32bit(ctx):
     """A config-4-like batch (lengths log-uniform in [256, 16384]) through the
     packed chain, bit for bit against the 32-bit register chain."""
