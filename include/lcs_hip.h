@@ -68,7 +68,8 @@ typedef struct lcs_schedule {
 #define LCS_VARIANT_CHAIN_LDS 2  /* band chain with b staged through LDS      */
 /* Batches only: pairs of a batch with this variant are combed two per wave
  * with 16-bit strands (pair X in the low, pair Y in the high half of every
- * register, 5 VALU per 2 cells against 4 per cell).  A pair qualifies when
+ * register, 5 VALU per 2 cells against 4 per cell; 4 per 2 cells when
+ * every symbol of both pairs lies in [0, 15]).  A pair qualifies when
  * m, n >= 1, m + n <= 65535 and every symbol lies in [0, 65533]; the others
  * (and an odd one out) take the register chain.  Same bits as every other
  * variant.  Elsewhere it means LCS_VARIANT_CHAIN. */
