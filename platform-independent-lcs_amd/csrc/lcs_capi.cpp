@@ -65,6 +65,26 @@ int default_tile_m(const lcs_ctx *ctx, int32_t m)
 
 bool valid_tile_m(int tm) { return tm == 1 || tm == 2 || tm == 4 || tm == 8 || tm == 16 || tm == 32; }
 
+// One strip whose bands cannot fill the SIMDs at one row per lane (m below
+// 64 rows per SIMD): the launch is the pipeline fill, (bands - 1) lags, plus
+// one band's sweep of n columns, so taller bands pay off unless the pair is
+// wide.  Measured on MI355X (one strip, byte symbols, m = 1024 .. 32768):
+// 4 rows per lane is fastest for n < 1.4 m (m = n = 4096: 0.301 ms against
+// 0.343 at one row, 0.321 at two), two rows between 1.4 m and 1.7 m by the
+// fitted per-height costs, and one row from n = 1.7 m (n = 4m: one row
+// 0.511 ms, four 0.722).  Taller bands are taken only where no partial band
+// would need the masked 32-bit loop (byte symbols pad their partial bands).
+int small_pair_tile_m(int variant, int32_t m, int32_t n)
+{
+    if (m < 1024) return 1;
+    const bool bytes = (variant == LCS_VARIANT_AUTO || variant == LCS_VARIANT_CHAIN ||
+                        variant == LCS_VARIANT_CHAIN_BYTES) && use_sym8(variant, m, n);
+    const long long n10 = 10LL * n;
+    if (n10 < 14LL * m && (bytes || m % 256 == 0)) return 4;
+    if (n10 < 17LL * m && (bytes || m % 128 == 0)) return 2;
+    return 1;
+}
+
 int32_t bands_of(int32_t m, int32_t n, int tm)
 {
     if (m <= 0 || n <= 0) return 0;
@@ -528,6 +548,8 @@ static int enqueue_chain_ws(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const i
                 tm = 16;
             else
                 tm = (s8 <= 32 && (long long)s8 * bands_of(m, n, 8) >= simds) ? 8 : 4;
+        } else if (tm == 1) {
+            tm = small_pair_tile_m(sched ? sched->variant : LCS_VARIANT_AUTO, m, n);
         }
     }
     int strips = strips_req ? strips_req : ((d_h_in || d_v_in) ? 1 : default_strips(ctx, m, n, tm));

@@ -98,6 +98,7 @@ struct lcs_ctx {
 
 namespace lcs {
 int default_tile_m(const lcs_ctx *ctx, int32_t m);
+int small_pair_tile_m(int variant, int32_t m, int32_t n);
 int enqueue_chain(lcs_ctx *ctx, const int32_t *d_a, int32_t m, const int32_t *d_b, int32_t n,
                   const lcs_schedule *sched, int32_t *d_h, int32_t *d_v, hipStream_t stream,
                   const int32_t *d_h_in = nullptr, const int32_t *d_v_in = nullptr);
