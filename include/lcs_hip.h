@@ -72,7 +72,9 @@ typedef struct lcs_schedule {
  * every symbol of both pairs lies in [0, 15]).  A pair qualifies when
  * m, n >= 1, m + n <= 65535 and every symbol lies in [0, 65533]; the others
  * (and an odd one out) take the register chain.  Same bits as every other
- * variant.  Elsewhere it means LCS_VARIANT_CHAIN. */
+ * variant.  A batch pair left to the library (tile_m 0, LCS_VARIANT_AUTO,
+ * after the context's table and cost model) that qualifies takes it too.
+ * Elsewhere it means LCS_VARIANT_CHAIN. */
 #define LCS_VARIANT_PACKED16 3
 /* LCS_VARIANT_CHAIN with the byte symbols staged whatever the pair's size
  * (single pairs; in a batch it means LCS_VARIANT_CHAIN). */

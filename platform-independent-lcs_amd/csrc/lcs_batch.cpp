@@ -106,6 +106,17 @@ int default_batch_tile_m(int32_t m)
     return 16;
 }
 
+// The packed chain's default for a pair left to the library: one band for a
+// short string, else 8 rows per lane below m = 8192 and 16 from there.  On
+// the 10,000-pair batch (m, n log-uniform in [256, 16384], alphabet 4) this
+// took 12.2-12.6 ms against 11.6-11.8 for the learned cost model's choice
+// and 27.5 for the register chain at default_batch_tile_m.
+static int default_batch_packed_tile_m(int32_t m)
+{
+    if (m <= 512) return default_batch_tile_m(m);
+    return m < 8192 ? 8 : 16;
+}
+
 // Runs the prepared launches side by side (fork/join over the class
 // streams), optionally captured in a hipGraph, between ev0 and ev1 on the
 // context stream.
@@ -199,7 +210,13 @@ int batch_core(lcs_ctx *ctx, Extras *X, int32_t npairs, const int32_t *da, const
     };
     for (int32_t p = 0; p < npairs; ++p) {
         int tm = (scheds && scheds[p].tile_m) ? scheds[p].tile_m : default_batch_tile_m(m[p]);
-        const int variant = scheds ? scheds[p].variant : LCS_VARIANT_AUTO;
+        int variant = scheds ? scheds[p].variant : LCS_VARIANT_AUTO;
+        if (variant == LCS_VARIANT_AUTO && !(scheds && (scheds[p].tile_m || scheds[p].strips > 1)) &&
+            packable(m[p], n[p])) {
+            // a pair left to the library that the packed chain can take
+            variant = LCS_VARIANT_PACKED16;
+            tm = default_batch_packed_tile_m(m[p]);
+        }
         const int c = class_of(tm, group_width(variant));
         if (c < 0) return LCS_EINVAL;
         if (scheds && scheds[p].waves_per_simd) {

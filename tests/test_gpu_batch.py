@@ -133,12 +133,17 @@ def test_packed16_onehot_and_xor_cells_match_oracle(ctx):
 
 def test_packed16_large_batch_equals_32bit(ctx):
     """A config-4-like batch (lengths log-uniform in [256, 16384]) through the
-    packed chain, bit for bit against the 32-bit register chain."""
+    packed chain, bit for bit against the 32-bit register chain, and the
+    library's own default (no schedules: the packed chain for every pair it
+    can take) equal to both."""
     pairs = lcs_amd.workload_batch(2000, seed=77)
     a_list, b_list = [p.a for p in pairs], [p.b for p in pairs]
-    h32, v32 = ctx.batch_comb(a_list, b_list)
+    h32, v32 = ctx.batch_comb(a_list, b_list, scheds=[lcs_amd.schedule(variant=lcs_amd.VARIANT_CHAIN)] * len(pairs))
     h16, v16 = ctx.batch_comb(a_list, b_list, scheds=[lcs_amd.schedule(variant=PACKED16)] * len(pairs))
     bad = [i for i in range(len(pairs)) if not (np.array_equal(h16[i], h32[i]) and np.array_equal(v16[i], v32[i]))]
+    assert not bad, bad[:5]
+    hd, vd = ctx.batch_comb(a_list, b_list)
+    bad = [i for i in range(len(pairs)) if not (np.array_equal(hd[i], h32[i]) and np.array_equal(vd[i], v32[i]))]
     assert not bad, bad[:5]
 
 
