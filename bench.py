@@ -179,9 +179,11 @@ def bench_batch(ctx, npairs: int):
     mutated at a per-pair rate in [5%, 20%] (20% of it indels), resident in
     HBM.  Plain path: one lcs_comb_device call per pair (the drop-in's own
     schedule).  Batched path: lcs_batch_comb_device, every pair in one
-    hipGraph launch, schedules from the built-in heuristic and from the
-    learned cost model.  Every batched result is compared with the plain
-    path's, pair by pair."""
+    hipGraph launch, schedules from the library's built-in default (no
+    model: the packed chain, 8 or 16 rows per lane by m, for every pair it
+    can take; reported as batched_heuristic_*) and from the learned cost
+    model.  Every batched result is compared with the plain path's, pair by
+    pair."""
     from lcs_amd import tuning
     L = lcs_amd.lib()
     t0 = time.perf_counter()
